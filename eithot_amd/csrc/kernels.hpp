@@ -1,0 +1,40 @@
+// Launch interface of the gfx950 pass kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "plan.hpp"
+
+namespace eithot {
+
+struct LaunchCfg {
+  int64_t lds_target = 64 * 1024;  // footprints are packed into one workgroup up to this
+  int threads = 256;
+  int xcd_remap = 1;               // keep consecutive footprints on one XCD
+};
+
+// One pass of `op` over the tensor at `data`, on `stream`.  Staged ops use
+// `scratch` (scratch_bytes >= one footprint).  Returns a hipError_t.
+hipError_t launch_op(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
+                     const LaunchCfg& cfg, hipStream_t stream);
+
+// Bytes of LDS the op's LDS kernel uses per workgroup with cfg (0 if staged).
+int64_t op_lds_bytes(const Op& op, const LaunchCfg& cfg);
+
+// Fill n elements of elem bytes with the counter-based pattern v(seed, first+i)
+// (mode 0: v = i, mode 1: splitmix64 of (seed, i)).
+hipError_t launch_fill(char* data, uint64_t n, int elem, uint64_t seed, int mode, uint64_t first,
+                       hipStream_t stream);
+
+// Count output elements that differ from the permuted pattern: out[dest(i)]
+// must equal v(seed, i) for every input linear index i.
+hipError_t launch_verify(const char* data, const int64_t* dims, const int* perm0, int rank,
+                         int elem, uint64_t seed, int mode, unsigned long long* d_mismatch,
+                         hipStream_t stream);
+
+// Raise the dynamic-LDS limit of every pass kernel to `bytes` (once per device).
+hipError_t configure_kernels(int64_t bytes);
+
+}  // namespace eithot

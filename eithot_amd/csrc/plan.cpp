@@ -1,0 +1,776 @@
+// Host-side planner: decomposer, stage lowering, estimators.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <map>
+#include <sstream>
+#include <tuple>
+
+#include "eithot_status.h"
+
+namespace eithot {
+
+int64_t gcd64(int64_t a, int64_t b) {
+  while (b) {
+    int64_t t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+int64_t modinv64(int64_t a, int64_t m) {
+  if (m <= 1) return 0;
+  // extended Euclid on (a mod m, m)
+  __int128 t = 0, nt = 1, r = m, nr = a % m;
+  while (nr != 0) {
+    __int128 q = r / nr;
+    __int128 tmp = t - q * nt;
+    t = nt;
+    nt = tmp;
+    tmp = r - q * nr;
+    r = nr;
+    nr = tmp;
+  }
+  if (t < 0) t += m;
+  return (int64_t)t;
+}
+
+// ============================================================================
+// Decomposer.
+//
+// The reference generates each round with permutation_util.cpp:300-374 and
+// picks pre- or post-order with Nd_transpose.cpp:146-157.  Written here as
+// its equivalent block form: a round takes the current dim order `cur`, finds
+// the anchor mismatch against the target, and swaps two adjacent blocks X|Y
+// -> Y|X; the untouched prefix P and suffix S are merged into single dims.
+//   pre-order  (permutation_util.cpp:146-180): u = first mismatch,
+//     X = cur[u, t), Y = the longest run cur[t, t+L) equal to target[u, u+L)
+//     where t is the position of target[u] in cur.
+//   post-order (permutation_util.cpp:183-230): u = last mismatch,
+//     Y = cur[t+1, u], X = the longest run cur[t-L+1, t] equal to
+//     target[u-L+1, u].
+// ============================================================================
+namespace {
+
+struct Round {
+  bool done = false;
+  Stage st;
+  std::vector<int> next;
+};
+
+Round make_round(bool post, const std::vector<int>& cur, const std::vector<int>& target,
+                 const int64_t* dims) {
+  const int r = (int)cur.size();
+  Round R;
+  int u = -1;
+  if (!post) {
+    for (int i = 0; i < r; ++i)
+      if (cur[i] != target[i]) { u = i; break; }
+  } else {
+    for (int i = r - 1; i >= 0; --i)
+      if (cur[i] != target[i]) { u = i; break; }
+  }
+  if (u < 0 || (post && u == 0)) {
+    R.done = true;
+    return R;
+  }
+  const int t = (int)(std::find(cur.begin(), cur.end(), target[u]) - cur.begin());
+  int x0, y0, y1;  // P = [0,x0) X = [x0,y0) Y = [y0,y1) S = [y1,r)
+  if (!post) {
+    int L = 1;
+    while (u + L < r && t + L < r && target[u + L] == cur[t + L]) ++L;
+    x0 = u;
+    y0 = t;
+    y1 = t + L;
+  } else {
+    int L = 1;
+    while (u - L >= 0 && t - L >= 0 && target[u - L] == cur[t - L]) ++L;
+    x0 = t - L + 1;
+    y0 = t + 1;
+    y1 = u + 1;
+  }
+  auto prod = [&](int lo, int hi) {
+    int64_t p = 1;
+    for (int i = lo; i < hi; ++i) p *= dims[cur[i] - 1];
+    return p;
+  };
+  Stage& s = R.st;
+  s.P = prod(0, x0);
+  s.X = prod(x0, y0);
+  s.Y = prod(y0, y1);
+  s.S = prod(y1, r);
+  const bool hasP = x0 > 0, hasS = y1 < r;
+  int n = 0;
+  if (hasP) s.dims[n++] = s.P;
+  s.dims[n++] = s.X;
+  s.dims[n++] = s.Y;
+  if (hasS) s.dims[n++] = s.S;
+  s.rank = n;
+  // output order P, Y, X, S expressed as 1-based positions of the input blocks
+  int xi = hasP ? 2 : 1, yi = xi + 1;
+  n = 0;
+  if (hasP) s.perm[n++] = 1;
+  s.perm[n++] = yi;
+  s.perm[n++] = xi;
+  if (hasS) s.perm[n++] = yi + 1;
+  R.next.reserve(r);
+  for (int i = 0; i < x0; ++i) R.next.push_back(cur[i]);
+  for (int i = y0; i < y1; ++i) R.next.push_back(cur[i]);
+  for (int i = x0; i < y0; ++i) R.next.push_back(cur[i]);
+  for (int i = y1; i < r; ++i) R.next.push_back(cur[i]);
+  return R;
+}
+
+// Nd_transpose.cpp:104-113: 1 when a stage cannot run as a single
+// linearized pass under the reference's thresholds.
+int not_linearizable(const Stage& s) {
+  if (s.rank == 3) {
+    if (s.perm[2] == 3 && s.dims[2] >= 8) return 0;
+    if (s.perm[0] == 1 && s.dims[0] >= 48) return 0;
+  } else if (s.rank == 4) {
+    if (s.perm[3] == 4 && s.dims[3] >= 4) return 0;
+    if (s.perm[0] == 1 && s.dims[0] >= 96) return 0;
+  }
+  return 1;
+}
+
+int count_not_linearizable(bool post, const std::vector<int>& target, const int64_t* dims) {
+  const int r = (int)target.size();
+  std::vector<int> cur(r);
+  for (int i = 0; i < r; ++i) cur[i] = i + 1;
+  int cnt = 0;
+  for (int round = 0; round < r; ++round) {
+    Round R = make_round(post, cur, target, dims);
+    if (R.done) break;
+    cnt += not_linearizable(R.st);
+    cur = R.next;
+    if (cur == target) break;
+  }
+  return cnt;
+}
+
+}  // namespace
+
+bool decompose(const int64_t* dims, const int* perm1, int rank, int reorder,
+               std::vector<Stage>& out) {
+  out.clear();
+  if (rank < 1 || rank > 64) return false;
+  std::vector<int> target(perm1, perm1 + rank), seen(rank + 1, 0);
+  for (int v : target) {
+    if (v < 1 || v > rank || seen[v]) return false;
+    seen[v] = 1;
+  }
+  bool post;
+  if (reorder == REORDER_PRE) post = false;
+  else if (reorder == REORDER_POST) post = true;
+  else post = !(count_not_linearizable(false, target, dims) < count_not_linearizable(true, target, dims));
+  std::vector<int> cur(rank);
+  for (int i = 0; i < rank; ++i) cur[i] = i + 1;
+  for (int round = 0; round < rank; ++round) {
+    // the reference stops as soon as either generator reports "matched"
+    Round pre = make_round(false, cur, target, dims);
+    Round pst = make_round(true, cur, target, dims);
+    if (pre.done || pst.done) break;
+    Round& R = post ? pst : pre;
+    out.push_back(R.st);
+    cur = R.next;
+    if (cur == target) break;
+  }
+  return true;
+}
+
+// ============================================================================
+// Lowering of one batched 2-D transpose (B batches of R x C units of U bytes,
+// C fastest) to device passes.
+// ============================================================================
+
+int64_t Op::footprint_units() const {
+  switch (kind) {
+    case OP_TILE: return R * C;
+    case OP_COL: return R * W;
+    case OP_ROW: return C;
+  }
+  return 0;
+}
+
+int64_t Op::num_footprints() const {
+  const int64_t slices = U / us;
+  switch (kind) {
+    case OP_TILE: return B * slices;
+    case OP_COL: return B * ((C + W - 1) / W) * slices;
+    case OP_ROW: return B * R * slices;
+  }
+  return 0;
+}
+
+namespace {
+
+int64_t pow2_factor(int64_t x, int64_t cap) {
+  int64_t w = 1;
+  while (w < cap && x % (w * 2) == 0) w *= 2;
+  return w;
+}
+
+// Largest slice of a U-byte unit such that `units` slices fit `budget`
+// bytes, keeping the unit's power-of-two alignment (so the word width of the
+// copy does not shrink) and at least `min_seg` contiguous bytes.  0 if none.
+int64_t pick_slice(int64_t units, int64_t U, int64_t budget, int64_t min_seg) {
+  if (units <= 0) return 0;
+  if (units * U <= budget) return U;
+  const int64_t align = pow2_factor(U, 16);
+  const int64_t cap = budget / units;
+  if (cap < min_seg) return 0;
+  int64_t best = 0;
+  // divisors of U that are multiples of `align`, at most cap
+  for (int64_t q = 1; q * q <= U; ++q) {
+    if (U % q) continue;
+    for (int64_t us : {q, U / q})
+      if (us % align == 0 && us <= cap && us >= min_seg && us > best) best = us;
+  }
+  return best;
+}
+
+struct CatPlan {
+  bool ok = false;
+  int64_t m, n, c, a, b;
+  int64_t col_W, col_us;
+  bool col_staged;
+  int64_t row_us;
+  bool row_staged;
+  int npass;
+  double cost;
+};
+
+// Catanzaro passes act on an m x n grid with m <= n (C2R on R x C when
+// C >= R, R2C on the C x R view otherwise): columns of m units, rows of n.
+CatPlan plan_catanzaro(int64_t R, int64_t C, int64_t U, const DeviceProfile& prof,
+                       int64_t scratch_budget) {
+  CatPlan p;
+  p.m = std::min(R, C);
+  p.n = std::max(R, C);
+  p.c = gcd64(p.m, p.n);
+  p.a = p.m / p.c;
+  p.b = p.n / p.c;
+  p.npass = p.c > 1 ? 3 : 2;
+  // column passes
+  p.col_staged = false;
+  if (U >= 128) {
+    p.col_W = 1;
+    p.col_us = pick_slice(p.m, U, prof.lds_max, prof.min_segment);
+    if (!p.col_us) {
+      p.col_staged = true;
+      p.col_us = U;
+    }
+  } else {
+    int64_t W = std::min<int64_t>(p.n, (128 + U - 1) / U);
+    if (p.m * W * U > prof.lds_max) W = std::min<int64_t>(p.n, (prof.min_segment + U - 1) / U);
+    if (p.m * W * U > prof.lds_max) {
+      W = std::min<int64_t>(p.n, (128 + U - 1) / U);
+      p.col_staged = true;
+    }
+    p.col_W = W;
+    p.col_us = U;
+  }
+  // row pass
+  p.row_staged = false;
+  p.row_us = pick_slice(p.n, U, prof.lds_max, prof.min_segment);
+  if (!p.row_us) {
+    p.row_staged = true;
+    p.row_us = U;
+  }
+  const double col_cost = p.col_staged ? 2.0 : 1.0, row_cost = p.row_staged ? 2.0 : 1.0;
+  p.cost = row_cost + col_cost * (p.npass - 1);
+  // a staged pass needs at least one footprint of scratch
+  const int64_t col_fp = p.m * p.col_W * p.col_us, row_fp = p.n * p.row_us;
+  p.ok = true;
+  if (p.col_staged && col_fp > scratch_budget) p.ok = false;
+  if (p.row_staged && row_fp > scratch_budget) p.ok = false;
+  // small penalty per staged pass so that equal-traffic LDS plans win ties
+  p.cost += 0.01 * ((p.col_staged ? p.npass - 1 : 0) + (p.row_staged ? 1 : 0));
+  return p;
+}
+
+std::vector<int64_t> divisors(int64_t x) {
+  std::vector<int64_t> d;
+  for (int64_t q = 1; q * q <= x; ++q)
+    if (x % q == 0) {
+      d.push_back(q);
+      if (q != x / q) d.push_back(x / q);
+    }
+  std::sort(d.begin(), d.end());
+  return d;
+}
+
+enum How { HOW_NONE = 0, HOW_TILE = 1, HOW_CAT = 2, HOW_SPLIT_R = 3, HOW_SPLIT_C = 4 };
+
+struct Choice {
+  int how = HOW_NONE;
+  int64_t f = 0;   // split factor (the fast part)
+  double cost = 1e30;
+};
+
+struct Lowerer {
+  const DeviceProfile& prof;
+  int64_t scratch_budget;
+  std::map<std::tuple<int64_t, int64_t, int64_t, int>, Choice> memo;
+
+  Lowerer(const DeviceProfile& p, int64_t budget) : prof(p), scratch_budget(budget) {}
+
+  Choice best(int64_t R, int64_t C, int64_t U, int depth) {
+    Choice ch;
+    if (R <= 1 || C <= 1) {
+      ch.how = HOW_NONE;
+      ch.cost = 0;
+      return ch;
+    }
+    auto key = std::make_tuple(R, C, U, depth);
+    auto it = memo.find(key);
+    if (it != memo.end()) return it->second;
+    if (pick_slice(R * C, U, prof.lds_max, prof.min_segment)) {
+      ch.how = HOW_TILE;
+      ch.cost = 1.0;
+      memo[key] = ch;
+      return ch;
+    }
+    CatPlan cp = plan_catanzaro(R, C, U, prof, scratch_budget);
+    if (cp.ok) {
+      ch.how = HOW_CAT;
+      ch.cost = cp.cost;
+    }
+    if (depth < 3 && ch.cost > 2.0 + 1e-9) {
+      // R = R1 * R0: (B*R1, R0, C, U) then (B, R1, C, R0*U)
+      for (int64_t r0 : divisors(R)) {
+        if (r0 <= 1 || r0 >= R) continue;
+        double c1 = best(r0, C, U, depth + 1).cost;
+        if (c1 >= ch.cost) continue;
+        double c2 = best(R / r0, C, r0 * U, depth + 1).cost;
+        if (c1 + c2 + 1e-3 < ch.cost) {
+          ch.how = HOW_SPLIT_R;
+          ch.f = r0;
+          ch.cost = c1 + c2 + 1e-3;
+        }
+      }
+      // C = C1 * C0: (B, R, C1, C0*U) then (B*C1, R, C0, U)
+      for (int64_t c0 : divisors(C)) {
+        if (c0 <= 1 || c0 >= C) continue;
+        double c2 = best(R, c0, U, depth + 1).cost;
+        if (c2 >= ch.cost) continue;
+        double c1 = best(R, C / c0, c0 * U, depth + 1).cost;
+        if (c1 + c2 + 1e-3 < ch.cost) {
+          ch.how = HOW_SPLIT_C;
+          ch.f = c0;
+          ch.cost = c1 + c2 + 1e-3;
+        }
+      }
+    }
+    memo[key] = ch;
+    return ch;
+  }
+
+  void emit(int64_t B, int64_t R, int64_t C, int64_t U, uint64_t base, int stage, int depth,
+            std::vector<Op>& ops) {
+    if (R <= 1 || C <= 1 || B <= 0) return;
+    Choice ch = best(R, C, U, depth);
+    switch (ch.how) {
+      case HOW_TILE: {
+        Op op;
+        op.kind = OP_TILE;
+        op.stage = stage;
+        op.base = base;
+        op.B = B;
+        op.R = R;
+        op.C = C;
+        op.U = U;
+        op.us = pick_slice(R * C, U, prof.lds_max, prof.min_segment);
+        ops.push_back(op);
+        return;
+      }
+      case HOW_CAT: {
+        CatPlan cp = plan_catanzaro(R, C, U, prof, scratch_budget);
+        const bool c2r = C >= R;
+        Op g;
+        g.stage = stage;
+        g.base = base;
+        g.B = B;
+        g.R = cp.m;  // pass grid: m rows of n units
+        g.C = cp.n;
+        g.U = U;
+        g.m = cp.m;
+        g.n = cp.n;
+        g.c = cp.c;
+        g.a = cp.a;
+        g.b = cp.b;
+        g.k = modinv64(cp.a % cp.b, cp.b);
+        g.binv = modinv64(cp.b % cp.a, cp.a);
+        Op col = g, row = g;
+        col.kind = OP_COL;
+        col.W = cp.col_W;
+        col.us = cp.col_us;
+        col.staged = cp.col_staged;
+        row.kind = OP_ROW;
+        row.us = cp.row_us;
+        row.staged = cp.row_staged;
+        if (c2r) {
+          if (cp.c > 1) {
+            col.map = MAP_ROT;
+            ops.push_back(col);
+          }
+          row.map = MAP_ROWG;
+          ops.push_back(row);
+          col.map = MAP_COLSH;
+          ops.push_back(col);
+        } else {
+          col.map = MAP_COLSHINV;
+          ops.push_back(col);
+          row.map = MAP_ROWD;
+          ops.push_back(row);
+          if (cp.c > 1) {
+            col.map = MAP_ROTINV;
+            ops.push_back(col);
+          }
+        }
+        return;
+      }
+      case HOW_SPLIT_R: {
+        const int64_t r0 = ch.f, r1 = R / r0;
+        emit(B * r1, r0, C, U, base, stage, depth + 1, ops);
+        emit(B, r1, C, r0 * U, base, stage, depth + 1, ops);
+        return;
+      }
+      case HOW_SPLIT_C: {
+        const int64_t c0 = ch.f, c1 = C / c0;
+        emit(B, R, c1, c0 * U, base, stage, depth + 1, ops);
+        emit(B * c1, R, c0, U, base, stage, depth + 1, ops);
+        return;
+      }
+      default: {
+        Op bad;  // unreachable with a positive scratch budget; flagged by make_plan
+        bad.kind = 0;
+        bad.stage = stage;
+        ops.push_back(bad);
+        return;
+      }
+    }
+  }
+};
+
+}  // namespace
+
+void lower_transpose2d(int64_t B, int64_t R, int64_t C, int64_t U, uint64_t base, int stage,
+                       const DeviceProfile& prof, std::vector<Op>& ops) {
+  Lowerer L(prof, INT64_MAX / 4);
+  L.emit(B, R, C, U, base, stage, 0, ops);
+}
+
+static int64_t staged_scratch(const Op& op, int64_t budget) {
+  if (!op.staged) return 0;
+  const int64_t fp = op.footprint_bytes();
+  const int64_t nfp = op.num_footprints();
+  int64_t g = std::max<int64_t>(1, budget / std::max<int64_t>(fp, 1));
+  g = std::min(g, nfp);
+  return g * fp;
+}
+
+int make_plan(const int64_t* dims_in, const int* perm1, int rank, size_t elem, int n_sub,
+              double alpha, const DeviceProfile& prof, int reorder, Plan& plan) {
+  plan = Plan();
+  if (rank < 1 || rank > 64 || !dims_in || !perm1) return EITHOT_ERR_INVALID;
+  if (elem == 0 || elem > (1u << 20)) return EITHOT_ERR_ELEM;
+  if (!(alpha > 0) || !std::isfinite(alpha)) return EITHOT_ERR_INVALID;
+  std::vector<int64_t> dims(dims_in, dims_in + rank);
+  uint64_t vol = 1;
+  for (int64_t d : dims) {
+    if (d < 1) return EITHOT_ERR_INVALID;
+    if (vol > (UINT64_MAX / 16) / (uint64_t)d) return EITHOT_ERR_INVALID;
+    vol *= (uint64_t)d;
+  }
+  plan.tensor_bytes = vol * elem;
+  std::vector<Stage> stages;
+  if (!decompose(dims.data(), perm1, rank, reorder, stages)) return EITHOT_ERR_INVALID;
+  plan.stages = stages;
+  const int64_t budget =
+      std::max<int64_t>((int64_t)(alpha * (double)plan.tensor_bytes), (int64_t)1);
+  Lowerer L(prof, budget);
+
+  // Sub-tensor partitioning along the largest dim (Nd_transpose.cpp:265-300):
+  // only when the caller asks for it and the dim exceeds floor(32/alpha).
+  int nb = 1, p = 0;
+  for (int i = 1; i < rank; ++i)
+    if (dims[i] > dims[p]) p = i;
+  const int64_t large = (int64_t)(32.0 * (1.0 / alpha));
+  if (n_sub > 1 && dims[p] > large && !stages.empty()) {
+    // the largest divisor of the split dim not above n_sub: every sub-tensor
+    // has the same shape, so no padding headroom is needed in the buffer
+    for (int q = n_sub; q >= 2; --q)
+      if (dims[p] % q == 0) { nb = q; break; }
+  }
+  plan.n_sub = nb;
+  if (nb > 1) {
+    plan.partition_dim = p;
+    int64_t ds = 1, nds = 1;
+    for (int i = 0; i <= p; ++i) ds *= dims[i];
+    for (int i = p + 1; i < rank; ++i) nds *= dims[i];
+    // partition (Nd_decompose.cpp:16-25,48-56): (ds/nb, nb, nds) -> 132
+    L.emit(1, nds, nb, (ds / nb) * (int64_t)elem, 0, -1, 0, plan.ops);
+    std::vector<int64_t> sub = dims;
+    sub[p] /= nb;
+    std::vector<Stage> sst;
+    decompose(sub.data(), perm1, rank, reorder, sst);
+    for (size_t s = 0; s < sst.size(); ++s) {
+      const Stage& st = sst[s];
+      // all nb sub-tensors are contiguous and share the plan: one batched op
+      L.emit(st.S * nb, st.Y, st.X, st.P * (int64_t)elem, 0, (int)s, 0, plan.ops);
+    }
+    plan.stages = sst;
+    // join (Nd_decompose.cpp:32-41,65-82): (ds'/nb, nds', nb) -> 132
+    int pp = 0;
+    for (int i = 0; i < rank; ++i)
+      if (perm1[i] - 1 == p) pp = i;
+    int64_t ds2 = 1, nds2 = 1;
+    for (int i = 0; i <= pp; ++i) ds2 *= dims[perm1[i] - 1];
+    for (int i = pp + 1; i < rank; ++i) nds2 *= dims[perm1[i] - 1];
+    L.emit(1, nb, nds2, (ds2 / nb) * (int64_t)elem, 0, -1, 0, plan.ops);
+  } else {
+    for (size_t s = 0; s < stages.size(); ++s) {
+      const Stage& st = stages[s];
+      L.emit(st.S, st.Y, st.X, st.P * (int64_t)elem, 0, (int)s, 0, plan.ops);
+    }
+  }
+  for (const Op& op : plan.ops) {
+    if (op.kind == 0) return EITHOT_ERR_ALPHA;
+    plan.scratch_bytes = std::max(plan.scratch_bytes, staged_scratch(op, budget));
+  }
+  return EITHOT_OK;
+}
+
+std::string Plan::describe() const {
+  std::ostringstream os;
+  static const char* kn[] = {"?", "tile", "col", "row"};
+  static const char* mn[] = {"-", "rot", "rotinv", "colsh", "colshinv", "rowg", "rowd"};
+  os << "stages:";
+  for (const Stage& s : stages) {
+    os << " ";
+    for (int i = 0; i < s.rank; ++i) os << s.perm[i];
+    os << "(";
+    for (int i = 0; i < s.rank; ++i) os << (i ? "," : "") << s.dims[i];
+    os << ")";
+  }
+  os << "\nn_sub=" << n_sub << " scratch=" << scratch_bytes << "\n";
+  for (const Op& op : ops) {
+    os << "  [" << op.stage << "] " << kn[op.kind] << "/" << mn[op.map] << " B=" << op.B
+       << " R=" << op.R << " C=" << op.C << " U=" << op.U << " us=" << op.us;
+    if (op.kind == OP_COL) os << " W=" << op.W;
+    if (op.staged) os << " STAGED";
+    os << "\n";
+  }
+  return os.str();
+}
+
+// ============================================================================
+// Reference-parity estimator: the size model of the Nd_estimation twin
+// (Nd_memory_estimation/src/{col_op.cu:15-176,row_op.cu:14-173,
+// 2dtranspose.cpp,3dtranspose.cpp,4dtranspose.cpp:116-143,Nd_decompose.cpp,
+// Nd_padding.cu:95-112,test_inplace.cpp:18-147}).  Only gmem-tier passes
+// need scratch; every other tier returns 0.
+// ============================================================================
+namespace {
+
+struct RefModel {
+  const RefProfile& p;
+  int64_t T;  // sizeof(T): 4 -> float, anything else -> double (transpose.cpp:10-18)
+
+  static int msb(int64_t x) {
+    int r = -1;
+    while (x > 0) {
+      ++r;
+      x >>= 1;
+    }
+    return r;
+  }
+  int64_t nthr(bool rotate, int64_t d1) const {
+    if (rotate) return 256;
+    return msb(d1) <= 18 ? 32 : 256;
+  }
+  // col_op.cu:162-176 (_2d) and :111-128 (_213)
+  int64_t col2d(bool rot, int64_t d1, int64_t d2) const {
+    if (p.smem_per_block / (T * d2) >= 16) return 0;
+    return T * nthr(rot, d1) * d2;
+  }
+  // col_op.cu:71-88 (_132)
+  int64_t col132(bool rot, int64_t d1, int64_t d2) const {
+    if (2 * T * d1 * d2 <= p.smem_per_block / 32) return 0;
+    if (T * d1 * d2 <= p.smem_per_block) return 0;
+    if (p.smem_per_block / (T * d2) >= 16) return 0;
+    return T * nthr(rot, d1) * d2;
+  }
+  // row_op.cu:130-173 (_2d), :95-128 (_213)
+  int64_t row2d(int64_t d1) const {
+    if (2 * d1 * T <= p.smem_per_block / 32) return 0;
+    if (T * d1 <= p.smem_per_block) return 0;
+    return T * d1;
+  }
+  // row_op.cu:52-81 (_132)
+  int64_t row132(int64_t d1, int64_t d3) const {
+    if (T * d1 * d3 <= p.smem_per_block) return 0;
+    return T * d1 * d3;
+  }
+  // 2dtranspose.cpp:21-94
+  int64_t t2d(int64_t d1, int64_t d2) const {
+    const int64_t c = gcd64(d1, d2);
+    if (d1 >= d2)
+      return std::max({c > 1 ? col2d(true, d1, d2) : 0, row2d(d1), col2d(false, d1, d2)});
+    // r2c(d2, d1): the kernels see d2 columns and d1 rows
+    return std::max({col2d(false, d2, d1), row2d(d2), c > 1 ? col2d(true, d2, d1) : 0});
+  }
+  // 3dtranspose.cpp:133-171 (213)
+  int64_t t213(int64_t d1, int64_t d2, int64_t d3) const {
+    if (d3 >= 8) return row2d(d1 * d2);
+    const int64_t c = gcd64(d1, d2);
+    if (d1 > d2)
+      return std::max({c > 1 ? col2d(true, d1, d2) : 0, row2d(d1), col2d(false, d1, d2)});
+    return std::max({col2d(false, d2, d1), row2d(d2), c > 1 ? col2d(true, d2, d1) : 0});
+  }
+  // 3dtranspose.cpp:175-228 (132)
+  int64_t t132(int64_t d1, int64_t d2, int64_t d3) const {
+    if (d1 >= 48) return col2d(false, d1, d2 * d3);
+    if (d1 > 12) {
+      if (d2 > d3) {  // c2r(d1, d3, d2): grid of d3 rows, d2 columns
+        const int64_t c = gcd64(d3, d2);
+        return std::max({c > 1 ? col132(true, d1, d3) : 0, row132(d1, d2), col132(false, d1, d3)});
+      }
+      const int64_t c = gcd64(d3, d2);  // r2c(d1, d2, d3)
+      return std::max({col132(false, d1, d2), row132(d1, d3), c > 1 ? col132(true, d1, d2) : 0});
+    }
+    if (d2 >= d3) return std::max(t2d(d1 * d2, d3), row2d(d3 * d1));
+    return std::max(row2d(d1 * d2), t2d(d2, d1 * d3));
+  }
+  // 4dtranspose.cpp:116-143 (the estimator's 1324 rule has no alpha term)
+  int64_t t1324(int64_t d1, int64_t d2, int64_t d3, int64_t d4) const {
+    if (d1 >= 96) return col2d(false, d1, d2 * d3 * d4);
+    if (d4 >= 4) return row2d(d1 * d2 * d3);
+    if (d2 >= d3) return std::max(t213(d1 * d2, d3, d4), t213(d3, d1, d2 * d4));
+    return std::max({row2d(d1 * d2), t2d(d2, d1 * d3 * d4), col2d(false, d1 * d3, d4 * d2)});
+  }
+  int64_t stage(const Stage& s) const {
+    const int64_t* x = s.dims;
+    int code = 0;
+    for (int i = 0; i < s.rank; ++i) code = code * 10 + s.perm[i];
+    switch (code) {
+      case 21: return t2d(x[0], x[1]);
+      case 213: return t213(x[0], x[1], x[2]);
+      case 132: return t132(x[0], x[1], x[2]);
+      case 1324: return t1324(x[0], x[1], x[2], x[3]);
+    }
+    return 0;
+  }
+  // Nd_transpose.cpp (estimator twin):156-249, max over rounds
+  int64_t series(const int64_t* dims, const int* perm1, int rank) const {
+    std::vector<Stage> st;
+    decompose(dims, perm1, rank, REORDER_MIX, st);
+    int64_t m = 0;
+    for (const Stage& s : st) m = std::max(m, stage(s));
+    return m;
+  }
+};
+
+}  // namespace
+
+int estimate_reference(const int64_t* dims, const int* perm1, int rank, size_t elem,
+                       double alpha, const RefProfile& prof, RefEstimate& out) {
+  out = RefEstimate();
+  if (rank < 1 || !dims || !perm1 || !(alpha > 0)) return EITHOT_ERR_INVALID;
+  RefModel M{prof, elem == 4 ? 4 : 8};
+  double vol = 1;
+  for (int i = 0; i < rank; ++i) vol *= (double)dims[i];
+  const double data = vol * (double)M.T;
+  int64_t t0 = M.series(dims, perm1, rank);
+  out.init_bytes = (double)t0;
+  // test_inplace.cpp:41-43: nb = 2^ceil(log2(floor(T0/bytes/alpha))), 0 -> 1
+  const int percent = (int)((double)t0 / data / alpha);
+  int nb = percent > 0 ? (int)std::pow(2.0, std::ceil(std::log2((double)percent))) : 0;
+  if (nb == 0) nb = 1;
+  if (t0 > 0) {  // :46-49
+    alpha = std::max(alpha, 1.0 / std::sqrt(vol / 32.0));
+    out.new_ratio = alpha;
+    out.printed_new_ratio = true;
+  }
+  const int64_t large = (int64_t)(32.0 / alpha);
+  int mp = 0;  // Nd_memory_estimation/src/util.cpp:86-94 last argmax
+  for (int i = 0; i < rank; ++i)
+    if (dims[i] >= dims[mp]) mp = i;
+  if (!(dims[mp] > large && nb > 1)) {  // :62-79
+    out.nb = nb;
+    out.alpha = std::ceil(alpha * 100.0) / 100.0;
+    return EITHOT_OK;
+  }
+  out.decomposed = true;
+  // the reference keeps these sizes in size_t, so halving truncates
+  const int64_t D = dims[mp];
+  const int64_t ivol = (int64_t)vol;
+  int64_t transpose_tmp = t0 / nb;
+  int64_t padding_tmp = 0;
+  if (D % nb != 0) {
+    const int64_t Dp = (D / nb + 1) * nb;
+    padding_tmp = ivol / D * (Dp - D) * M.T;
+    const int64_t pre = M.T * prof.padding_blocks_per_sm * prof.sms * 1024;
+    padding_tmp = std::max(padding_tmp, pre);
+  }
+  // partition and join sizes (Nd_decompose.cpp:17-75 of the estimator)
+  int64_t ds = 1, nds = 1;
+  for (int i = 0; i <= mp; ++i) ds *= dims[i];
+  for (int i = mp + 1; i < rank; ++i) nds *= dims[i];
+  const int p132[3] = {1, 3, 2};
+  int64_t pd[3] = {ds / nb, nb, nds};
+  int64_t decompose_tmp = M.series(pd, p132, 3);
+  int pp = 0;
+  for (int i = 0; i < rank; ++i)
+    if (perm1[i] - 1 == mp) pp = i;
+  int64_t ds2 = 1, nds2 = 1;
+  for (int i = 0; i <= pp; ++i) ds2 *= dims[perm1[i] - 1];
+  for (int i = pp + 1; i < rank; ++i) nds2 *= dims[perm1[i] - 1];
+  int64_t jd[3] = {ds2 / nb, nds2, nb};
+  decompose_tmp = std::max(decompose_tmp, M.series(jd, p132, 3));
+  int64_t final_tmp = padding_tmp + std::max(decompose_tmp, transpose_tmp);
+  double ratio = (double)final_tmp / data;
+  if (ratio > alpha) {  // :97-122 halve nb while that does not increase the total
+    int64_t new_final = final_tmp;
+    int new_nb = nb;
+    while (final_tmp >= new_final && nb > 1) {
+      nb = new_nb;
+      final_tmp = new_final;
+      new_nb /= 2;
+      // the reference grows the dim even when it divides evenly (:107-108)
+      const int64_t Dp = (D / new_nb + 1) * new_nb;
+      padding_tmp = ivol / D * (Dp - D) * M.T;
+      transpose_tmp *= 2;
+      decompose_tmp /= 2;
+      new_final = padding_tmp + std::max(decompose_tmp, transpose_tmp);
+    }
+    ratio = (double)final_tmp / data;
+    ratio = std::ceil(ratio * 100.0) / 100.0;
+    alpha = std::max(ratio, alpha);
+  }
+  out.nb = nb;
+  out.final_bytes = (double)final_tmp;
+  out.final_ratio = ratio;
+  out.alpha = alpha;
+  return EITHOT_OK;
+}
+
+int estimate_native(const int64_t* dims, const int* perm1, int rank, size_t elem, double alpha,
+                    const DeviceProfile& prof, int* n_sub, double* alpha_out,
+                    size_t* extra_bytes) {
+  Plan plan;
+  int rc = make_plan(dims, perm1, rank, elem, 1, alpha, prof, REORDER_MIX, plan);
+  if (rc != EITHOT_OK) return rc;
+  if (n_sub) *n_sub = plan.n_sub;
+  if (extra_bytes) *extra_bytes = (size_t)plan.scratch_bytes;
+  if (alpha_out)
+    *alpha_out = plan.tensor_bytes ? (double)plan.scratch_bytes / (double)plan.tensor_bytes : 0.0;
+  return EITHOT_OK;
+}
+
+}  // namespace eithot

@@ -1,0 +1,129 @@
+// Host-side planner for in-place tensor transposition on MI355X.
+//
+// Pipeline: N-d permutation -> decomposer rounds (low-order stages, each a
+// swap of two adjacent dim blocks) -> every stage is a batched 2-D transpose
+// of U-byte units -> lowered to device passes (one-pass LDS tile transpose,
+// Catanzaro C2R/R2C column/row passes, or hierarchical splits) with a cost
+// model written for gfx950 (160 KB LDS per CU, 128-byte lines).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace eithot {
+
+// ---- decomposer ------------------------------------------------------------
+// One low-order stage: the tensor viewed as (P, X, Y, S) [fastest first]
+// becomes (P, Y, X, S).  `rank`/`perm`/`dims` give the reference's low-order
+// form (21, 213, 132 or 1324 with merged dims).
+struct Stage {
+  int rank = 0;
+  int perm[4] = {0, 0, 0, 0};
+  int64_t dims[4] = {1, 1, 1, 1};
+  int64_t P = 1, X = 1, Y = 1, S = 1;
+};
+
+enum Reorder { REORDER_MIX = 0, REORDER_PRE = 1, REORDER_POST = 2 };
+
+// perm1 is 1-based, dims fastest first.  Returns false on an invalid perm.
+bool decompose(const int64_t* dims, const int* perm1, int rank, int reorder,
+               std::vector<Stage>& out);
+
+// ---- device passes ---------------------------------------------------------
+enum OpKind : int { OP_TILE = 1, OP_COL = 2, OP_ROW = 3 };
+
+// Index maps of the passes, all in gather form (dest <- source).
+enum MapKind : int {
+  MAP_NONE = 0,
+  MAP_ROT = 1,         // C2R column rotate:   i' = (i + floor(j/b)) mod m
+  MAP_ROTINV = 2,      // R2C column rotate:   i' = (i - floor(j/b)) mod m
+  MAP_COLSH = 3,       // C2R column shuffle:  i' = (i*n + j - floor(i/a)) mod m
+  MAP_COLSHINV = 4,    // R2C column shuffle:  inverse of MAP_COLSH
+  MAP_ROWG = 5,        // C2R row shuffle (gather form)
+  MAP_ROWD = 6,        // R2C row shuffle (gather form)
+};
+
+struct Op {
+  int kind = 0;
+  int map = MAP_NONE;
+  int staged = 0;      // 1: footprint exceeds LDS, executed through scratch
+  int stage = -1;      // index of the decomposer stage it belongs to (-1: partition/join)
+  uint64_t base = 0;   // byte offset of batch 0
+  int64_t B = 1, R = 1, C = 1;  // B batches of an R x C grid of units
+  int64_t U = 1;       // unit bytes
+  int64_t us = 1;      // slice bytes (U % us == 0)
+  int64_t W = 1;       // OP_COL: columns per panel
+  // Catanzaro constants of the m x n grid (m = R, n = C of this op's grid)
+  int64_t m = 1, n = 1, c = 1, a = 1, b = 1, k = 0, binv = 0;
+
+  int64_t footprint_units() const;   // units staged per footprint
+  int64_t footprint_bytes() const { return footprint_units() * us; }
+  int64_t num_footprints() const;
+  uint64_t bytes() const { return (uint64_t)B * R * C * U; }
+};
+
+struct DeviceProfile {
+  int64_t lds_max = 160 * 1024;   // bytes of LDS one workgroup may use (gfx950)
+  int64_t lds_target = 64 * 1024; // per-workgroup LDS the launcher packs footprints to
+  int cus = 256;
+  int64_t min_segment = 64;       // smallest contiguous run worth a footprint slice
+};
+
+struct Plan {
+  std::vector<Stage> stages;
+  std::vector<Op> ops;
+  int n_sub = 1;              // sub-tensors actually used
+  int partition_dim = -1;     // 0-based input dim split into n_sub blocks
+  int64_t scratch_bytes = 0;  // device scratch the plan needs (0 if all LDS)
+  uint64_t tensor_bytes = 0;
+  std::string describe() const;
+};
+
+// Full plan.  perm1 is 1-based.  alpha bounds the scratch (scratch <= alpha *
+// tensor bytes, at least one footprint); n_sub > 1 requests the reference's
+// sub-tensor partitioning along the largest dim when that dim exceeds
+// floor(32/alpha) and divides evenly.  Returns 0 or a negative EITHOT_ERR_*.
+int make_plan(const int64_t* dims, const int* perm1, int rank, size_t elem, int n_sub,
+              double alpha, const DeviceProfile& prof, int reorder, Plan& plan);
+
+// Lowering of one batched 2-D transpose (exposed for tests).
+void lower_transpose2d(int64_t B, int64_t R, int64_t C, int64_t U, uint64_t base, int stage,
+                       const DeviceProfile& prof, std::vector<Op>& ops);
+
+// ---- estimators -------------------------------------------------------------
+// Reference-parity model of Nd_estimation (Nd_memory_estimation/src/
+// test_inplace.cpp:18-147) on a CUDA device profile.
+struct RefProfile {
+  int64_t smem_per_block = 49152;   // RTX 3090
+  int sms = 82;
+  int max_threads_per_sm = 1536;
+  int padding_blocks_per_sm = 1;    // occupancy of the 1024-thread padding kernel
+};
+
+struct RefEstimate {
+  double init_bytes = 0;     // "init Extra Memory size"
+  double new_ratio = 0;      // "new ratio" (only printed when init_bytes > 0)
+  bool printed_new_ratio = false;
+  bool decomposed = false;   // "Final Extra Memory Size" line printed
+  double final_bytes = 0;
+  double final_ratio = 0;
+  int nb = 1;
+  double alpha = 0;          // the alpha written to proper_input.txt
+};
+
+int estimate_reference(const int64_t* dims, const int* perm1, int rank, size_t elem,
+                       double alpha, const RefProfile& prof, RefEstimate& out);
+
+// Native estimate for this build's kernels: scratch bytes of the plan at the
+// given alpha, the sub-tensor count, and the resulting extra-memory ratio.
+int estimate_native(const int64_t* dims, const int* perm1, int rank, size_t elem, double alpha,
+                    const DeviceProfile& prof, int* n_sub, double* alpha_out,
+                    size_t* extra_bytes);
+
+// ---- integer helpers ----------------------------------------------------------
+int64_t gcd64(int64_t a, int64_t b);
+int64_t modinv64(int64_t a, int64_t m);  // a^{-1} mod m (0 when m == 1)
+
+}  // namespace eithot

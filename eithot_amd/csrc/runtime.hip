@@ -1,0 +1,395 @@
+// C-ABI of libeithot.so: device contexts, scratch arena, plan cache and the
+// synchronous / stream-ordered transpose entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/eithot.h"
+#include "kernels.hpp"
+#include "plan.hpp"
+#include "runtime.hpp"
+
+namespace eithot {
+
+namespace {
+thread_local std::string g_err;
+thread_local size_t g_peak = 0;
+thread_local std::vector<float> g_pass_ms;
+int g_timing = 0;
+}  // namespace
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+void set_peak(size_t b) { g_peak = b; }
+
+Device* device_ctx(int* err) {
+  static std::mutex mu;
+  static std::vector<std::unique_ptr<Device>> devs;
+  int id = 0;
+  if (hipGetDevice(&id) != hipSuccess) {
+    if (err) *err = fail(EITHOT_ERR_NO_DEVICE, "no HIP device available");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)devs.size() <= id) devs.resize(id + 1);
+  if (!devs[id]) {
+    auto d = std::make_unique<Device>();
+    d->id = id;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, id) != hipSuccess) {
+      if (err) *err = fail(EITHOT_ERR_NO_DEVICE, "hipGetDeviceProperties failed");
+      return nullptr;
+    }
+    d->prof.cus = prop.multiProcessorCount;
+    // one workgroup may use the whole LDS of a CU on gfx950 (160 KiB)
+    int64_t lds = (int64_t)prop.maxSharedMemoryPerMultiProcessor;
+    if (lds <= 0) lds = (int64_t)prop.sharedMemPerBlock;
+    d->prof.lds_max = std::min<int64_t>(lds, 160 * 1024);
+    d->prof.lds_target = std::min<int64_t>(64 * 1024, d->prof.lds_max);
+    hipError_t e = configure_kernels(d->prof.lds_max);
+    if (e != hipSuccess) {
+      // fall back to the per-block limit the runtime reports
+      d->prof.lds_max = (int64_t)prop.sharedMemPerBlock;
+      d->prof.lds_target = std::min<int64_t>(d->prof.lds_target, d->prof.lds_max);
+      (void)hipGetLastError();
+    }
+    d->cfg.lds_target = d->prof.lds_target;
+    devs[id] = std::move(d);
+  }
+  return devs[id].get();
+}
+
+const Plan* Device::plan_for(const int64_t* dims, const int* perm1, int rank, size_t elem,
+                             int n_sub, double alpha, int* rc) {
+  std::ostringstream k;
+  for (int i = 0; i < rank; ++i) k << dims[i] << ",";
+  k << "|";
+  for (int i = 0; i < rank; ++i) k << perm1[i] << ",";
+  k << "|" << elem << "|" << n_sub << "|" << alpha;
+  const std::string key = k.str();
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = plans.find(key);
+  if (it != plans.end()) {
+    *rc = EITHOT_OK;
+    return it->second.get();
+  }
+  auto p = std::make_unique<Plan>();
+  *rc = make_plan(dims, perm1, rank, elem, n_sub, alpha, prof, REORDER_MIX, *p);
+  if (*rc != EITHOT_OK) return nullptr;
+  if (plans.size() > 4096) plans.clear();
+  const Plan* raw = p.get();
+  plans[key] = std::move(p);
+  return raw;
+}
+
+int Device::reserve_arena(int64_t bytes, hipStream_t stream) {
+  if (bytes <= 0) return EITHOT_OK;
+  std::lock_guard<std::mutex> lk(mu);
+  if (arena_ready) {  // earlier async users of the arena must finish first
+    if (hipStreamWaitEvent(stream, arena_ready, 0) != hipSuccess)
+      return fail(EITHOT_ERR_DEVICE, "hipStreamWaitEvent failed");
+  }
+  if (arena_bytes >= bytes) return EITHOT_OK;
+  if (arena) {
+    hipDeviceSynchronize();
+    hipFree(arena);
+    arena = nullptr;
+    arena_bytes = 0;
+  }
+  if (hipMalloc((void**)&arena, (size_t)bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    arena = nullptr;
+    return fail(EITHOT_ERR_NOMEM, "scratch allocation of %lld bytes failed", (long long)bytes);
+  }
+  arena_bytes = bytes;
+  return EITHOT_OK;
+}
+
+int Device::release_arena_after(hipStream_t stream) {
+  std::lock_guard<std::mutex> lk(mu);
+  if (!arena_ready && hipEventCreateWithFlags(&arena_ready, hipEventDisableTiming) != hipSuccess)
+    return fail(EITHOT_ERR_DEVICE, "hipEventCreate failed");
+  if (hipEventRecord(arena_ready, stream) != hipSuccess)
+    return fail(EITHOT_ERR_DEVICE, "hipEventRecord failed");
+  return EITHOT_OK;
+}
+
+int run_plan(Device* d, const Plan& plan, char* data, char* scratch, int64_t scratch_bytes,
+             hipStream_t stream, std::vector<float>* times) {
+  std::vector<hipEvent_t> ev;
+  if (times) {
+    ev.resize(plan.ops.size() + 1);
+    for (auto& e : ev) hipEventCreate(&e);
+    hipEventRecord(ev[0], stream);
+  }
+  int rc = EITHOT_OK;
+  for (size_t i = 0; i < plan.ops.size(); ++i) {
+    hipError_t e = launch_op(plan.ops[i], data, scratch, scratch_bytes, d->cfg, stream);
+    if (e != hipSuccess) {
+      rc = fail(EITHOT_ERR_DEVICE, "pass %zu launch failed: %s", i, hipGetErrorString(e));
+      break;
+    }
+    if (times) hipEventRecord(ev[i + 1], stream);
+  }
+  if (times) {
+    times->clear();
+    if (rc == EITHOT_OK) {
+      hipEventSynchronize(ev.back());
+      for (size_t i = 0; i < plan.ops.size(); ++i) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+        times->push_back(ms);
+      }
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+  }
+  return rc;
+}
+
+int validate(const void* dev, const int64_t* dims, const int* perm, int order, size_t elem) {
+  if (!dev) return fail(EITHOT_ERR_INVALID, "null tensor pointer");
+  if (!dims || !perm || order < 1 || order > 64)
+    return fail(EITHOT_ERR_INVALID, "order must be in 1..64 with dims and perm given");
+  if (elem != 1 && elem != 2 && elem != 4 && elem != 8)
+    return fail(EITHOT_ERR_ELEM, "elem_bytes must be 1, 2, 4 or 8 (got %zu)", elem);
+  std::vector<int> seen(order + 1, 0);
+  for (int i = 0; i < order; ++i) {
+    if (dims[i] < 1) return fail(EITHOT_ERR_INVALID, "dims[%d] = %lld < 1", i, (long long)dims[i]);
+    if (perm[i] < 1 || perm[i] > order || seen[perm[i]])
+      return fail(EITHOT_ERR_INVALID, "perm is not a 1-based permutation of 1..%d", order);
+    seen[perm[i]] = 1;
+  }
+  if (((uintptr_t)dev) % elem) return fail(EITHOT_ERR_INVALID, "tensor pointer misaligned");
+  return EITHOT_OK;
+}
+
+static int transpose_impl(void* dev, const int64_t* dims, const int* perm, int order, size_t elem,
+                          int n_sub, double alpha, hipStream_t stream, bool sync) {
+  g_err.clear();
+  g_peak = 0;
+  int rc = validate(dev, dims, perm, order, elem);
+  if (rc) return rc;
+  if (!(alpha > 0)) return fail(EITHOT_ERR_INVALID, "alpha must be > 0");
+  Device* d = device_ctx(&rc);
+  if (!d) return rc;
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(EITHOT_ERR_INVALID, "pointer is not device memory");
+  }
+  const Plan* plan = d->plan_for(dims, perm, order, elem, n_sub, alpha, &rc);
+  if (!plan) {
+    if (!g_err.size()) fail(rc, "planning failed (%d)", rc);
+    return rc;
+  }
+  if (plan->scratch_bytes > 0) {
+    rc = d->reserve_arena(plan->scratch_bytes, stream);
+    if (rc) return rc;
+  }
+  std::vector<float>* times = g_timing ? &g_pass_ms : nullptr;
+  rc = run_plan(d, *plan, (char*)dev, d->arena, plan->scratch_bytes, stream, times);
+  if (rc) return rc;
+  if (plan->scratch_bytes > 0) {
+    rc = d->release_arena_after(stream);
+    if (rc) return rc;
+  }
+  g_peak = (size_t)plan->scratch_bytes;
+  if (sync) {
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) return fail(EITHOT_ERR_DEVICE, "device error: %s", hipGetErrorString(e));
+  }
+  return EITHOT_OK;
+}
+
+}  // namespace eithot
+
+using namespace eithot;
+
+extern "C" {
+
+int eithot_transpose(void* dev, const int64_t* dims, const int* perm, int order,
+                     size_t elem_bytes, int n_sub, double alpha) {
+  return transpose_impl(dev, dims, perm, order, elem_bytes, n_sub, alpha, nullptr, true);
+}
+
+int eithot_transpose_async(void* dev, const int64_t* dims, const int* perm, int order,
+                           size_t elem_bytes, int n_sub, double alpha, void* stream) {
+  return transpose_impl(dev, dims, perm, order, elem_bytes, n_sub, alpha, (hipStream_t)stream,
+                        false);
+}
+
+int eithot_inplace_transpose(void* data, int source, int rank, const int* dim,
+                             const int* permutation, size_t sizeofType, int num_block,
+                             double large_ratio) {
+  if (rank < 1 || rank > 64 || !dim || !permutation || source < 0)
+    return fail(EITHOT_ERR_INVALID, "invalid rank, dims, permutation or source");
+  std::vector<int64_t> d(rank);
+  std::vector<int> p(rank);
+  for (int i = 0; i < rank; ++i) {
+    d[i] = dim[i];
+    p[i] = permutation[i] + 1;
+  }
+  char* base = (char*)data + (size_t)source * sizeofType;
+  return eithot_transpose(base, d.data(), p.data(), rank, sizeofType, num_block, large_ratio);
+}
+
+size_t eithot_last_peak_extra_bytes(void) { return g_peak; }
+
+const char* eithot_last_error(void) { return g_err.c_str(); }
+
+void eithot_release(void) {
+  int err = 0;
+  Device* d = device_ctx(&err);
+  if (!d) return;
+  std::lock_guard<std::mutex> lk(d->mu);
+  hipDeviceSynchronize();
+  if (d->arena) hipFree(d->arena);
+  d->arena = nullptr;
+  d->arena_bytes = 0;
+  d->plans.clear();
+}
+
+int eithot_estimate(const int64_t* dims, const int* perm, int order, size_t elem_bytes,
+                    double alpha, int* n_sub_out, double* alpha_out, size_t* extra_bytes_out) {
+  DeviceProfile prof;
+  int rc = estimate_native(dims, perm, order, elem_bytes, alpha, prof, n_sub_out, alpha_out,
+                           extra_bytes_out);
+  if (rc) return fail(rc, "estimate failed (%d)", rc);
+  return EITHOT_OK;
+}
+
+int eithot_estimate_reference(const int64_t* dims, const int* perm, int order, size_t elem_bytes,
+                              double alpha, const eithot_ref_profile* prof, double* init_bytes,
+                              double* new_ratio, double* final_bytes, double* final_ratio,
+                              int* nb, double* alpha_out, int* flags) {
+  RefProfile p;
+  if (prof) {
+    p.smem_per_block = prof->smem_per_block;
+    p.sms = prof->sms;
+    p.max_threads_per_sm = prof->max_threads_per_sm;
+    p.padding_blocks_per_sm = prof->padding_blocks_per_sm;
+  }
+  RefEstimate e;
+  int rc = estimate_reference(dims, perm, order, elem_bytes, alpha, p, e);
+  if (rc) return fail(rc, "reference estimate failed (%d)", rc);
+  if (init_bytes) *init_bytes = e.init_bytes;
+  if (new_ratio) *new_ratio = e.new_ratio;
+  if (final_bytes) *final_bytes = e.final_bytes;
+  if (final_ratio) *final_ratio = e.final_ratio;
+  if (nb) *nb = e.nb;
+  if (alpha_out) *alpha_out = e.alpha;
+  if (flags) *flags = (e.printed_new_ratio ? 1 : 0) | (e.decomposed ? 2 : 0);
+  return EITHOT_OK;
+}
+
+int eithot_decompose(const int64_t* dims, const int* perm, int order, int reorder,
+                     int* stage_rank, int* stage_perm, int64_t* stage_dims, int max_stages) {
+  std::vector<Stage> st;
+  if (!dims || !perm || !decompose(dims, perm, order, reorder, st))
+    return fail(EITHOT_ERR_INVALID, "invalid permutation");
+  const int n = (int)st.size();
+  for (int s = 0; s < n && s < max_stages; ++s) {
+    if (stage_rank) stage_rank[s] = st[s].rank;
+    for (int i = 0; i < 4; ++i) {
+      if (stage_perm) stage_perm[4 * s + i] = st[s].perm[i];
+      if (stage_dims) stage_dims[4 * s + i] = st[s].dims[i];
+    }
+  }
+  return n;
+}
+
+int eithot_plan_describe(const int64_t* dims, const int* perm, int order, size_t elem_bytes,
+                         int n_sub, double alpha, int64_t lds_max, eithot_op_desc* ops,
+                         int max_ops, int64_t* scratch_bytes) {
+  DeviceProfile prof;
+  if (lds_max > 0) {
+    prof.lds_max = lds_max;
+    prof.lds_target = std::min<int64_t>(prof.lds_target, lds_max);
+  }
+  Plan plan;
+  int rc = make_plan(dims, perm, order, elem_bytes, n_sub, alpha, prof, REORDER_MIX, plan);
+  if (rc) return fail(rc, "planning failed (%d)", rc);
+  if (scratch_bytes) *scratch_bytes = plan.scratch_bytes;
+  const int n = (int)plan.ops.size();
+  for (int i = 0; i < n && i < max_ops && ops; ++i) {
+    const Op& o = plan.ops[i];
+    eithot_op_desc& x = ops[i];
+    x.kind = o.kind;
+    x.map = o.map;
+    x.staged = o.staged;
+    x.stage = o.stage;
+    x.base = o.base;
+    x.B = o.B;
+    x.R = o.R;
+    x.C = o.C;
+    x.U = o.U;
+    x.us = o.us;
+    x.W = o.W;
+    x.m = o.m;
+    x.n = o.n;
+    x.c = o.c;
+    x.a = o.a;
+    x.b = o.b;
+    x.k = o.k;
+    x.binv = o.binv;
+  }
+  return n;
+}
+
+void eithot_set_pass_timing(int enable) { g_timing = enable ? 1 : 0; }
+
+int eithot_last_pass_times(float* ms, int max) {
+  const int n = (int)g_pass_ms.size();
+  for (int i = 0; i < n && i < max && ms; ++i) ms[i] = g_pass_ms[i];
+  return n;
+}
+
+int eithot_fill_pattern(void* dev, uint64_t n, size_t elem_bytes, uint64_t seed, int mode,
+                        uint64_t first, void* stream) {
+  if (!dev) return fail(EITHOT_ERR_INVALID, "null pointer");
+  hipError_t e = launch_fill((char*)dev, n, (int)elem_bytes, seed, mode, first, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(EITHOT_ERR_DEVICE, "fill failed: %s", hipGetErrorString(e));
+  return EITHOT_OK;
+}
+
+int eithot_verify_pattern(const void* dev, const int64_t* dims, const int* perm, int order,
+                          size_t elem_bytes, uint64_t seed, int mode,
+                          unsigned long long* mismatches) {
+  int rc = validate(dev, dims, perm, order, elem_bytes);
+  if (rc) return rc;
+  if (order > 16) return fail(EITHOT_ERR_UNSUPPORTED, "verify supports order <= 16");
+  std::vector<int> p0(order);
+  for (int i = 0; i < order; ++i) p0[i] = perm[i] - 1;
+  unsigned long long* d_cnt = nullptr;
+  if (hipMalloc((void**)&d_cnt, sizeof(unsigned long long)) != hipSuccess)
+    return fail(EITHOT_ERR_NOMEM, "counter allocation failed");
+  hipMemset(d_cnt, 0, sizeof(unsigned long long));
+  hipError_t e = launch_verify((const char*)dev, dims, p0.data(), order, (int)elem_bytes, seed,
+                               mode, d_cnt, nullptr);
+  unsigned long long h = 0;
+  if (e == hipSuccess) e = hipMemcpy(&h, d_cnt, sizeof(h), hipMemcpyDeviceToHost);
+  hipFree(d_cnt);
+  if (e != hipSuccess) return fail(EITHOT_ERR_DEVICE, "verify failed: %s", hipGetErrorString(e));
+  if (mismatches) *mismatches = h;
+  return EITHOT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,186 @@
+/*
+ * eithot -- in-place high-order tensor transposition on MI355X (gfx950).
+ *
+ * C-ABI of libeithot.so: plain pointers and sizes, no HIP or torch types.
+ * Streams are passed as `void*` (a hipStream_t, or NULL for the null stream).
+ *
+ * Conventions (ITNT/src/Nd_transpose.cpp:43-58, ITNT/src/tensor_util.cpp:91-104):
+ *   dims[0] is the fastest-varying dimension.  Output dim k is input dim
+ *   perm[k]; `perm` is 1-based in this ABI (as on the reference CLI,
+ *   ITNT/src/test_inplace.cpp:155), except in eithot_inplace_transpose which
+ *   keeps the reference library's 0-based signature.
+ *   Element bytes 1, 2, 4 or 8 are moved, never interpreted.
+ *
+ * Every entry point returns EITHOT_OK (0) or a negative EITHOT_ERR_* code and
+ * never exits the process; eithot_last_error() returns the message of the
+ * calling thread's last failure.  After EITHOT_ERR_DEVICE the tensor's
+ * contents are indeterminate; after any other error it is untouched.
+ */
+#ifndef EITHOT_H
+#define EITHOT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EITHOT_OK 0
+#define EITHOT_ERR_INVALID (-1)
+#define EITHOT_ERR_ELEM (-2)
+#define EITHOT_ERR_ALPHA (-3)
+#define EITHOT_ERR_DEVICE (-4)
+#define EITHOT_ERR_NOMEM (-5)
+#define EITHOT_ERR_UNSUPPORTED (-6)
+#define EITHOT_ERR_NO_DEVICE (-7)
+#define EITHOT_ERR_BUSY (-8)
+
+/* ---- the hot path --------------------------------------------------------- */
+
+/* Transposes the tensor at device pointer `dev` in place and returns when it
+ * is done.  n_sub > 1 asks for the reference's sub-tensor partitioning along
+ * the largest dim (applied when that dim exceeds floor(32/alpha); the count
+ * used is the largest divisor of the dim not above n_sub); n_sub <= 0 takes
+ * the estimator's choice.  Extra device memory stays <= alpha * tensor bytes.
+ * Replaces inplace::transpose (ITNT/include/transpose.h:8,
+ * ITNT/src/transpose.cpp:21-30). */
+int eithot_transpose(void* dev, const int64_t* dims, const int* perm, int order,
+                     size_t elem_bytes, int n_sub, double alpha);
+
+/* Same, enqueued on `stream` without a host synchronisation.  The scratch
+ * arena stays reserved until the work completes; do not free `dev` first. */
+int eithot_transpose_async(void* dev, const int64_t* dims, const int* perm, int order,
+                           size_t elem_bytes, int n_sub, double alpha, void* stream);
+
+/* The reference library's own signature and semantics: 0-based permutation,
+ * int dims, element offset `source`, sizeofType 1/2/4/8, num_block and the
+ * extra-memory ratio.  Replaces inplace::transpose
+ * (ITNT/include/transpose.h:8). */
+int eithot_inplace_transpose(void* data, int source, int rank, const int* dim,
+                             const int* permutation, size_t sizeofType, int num_block,
+                             double large_ratio);
+
+/* Peak extra device bytes (scratch) used by the calling thread's last
+ * transpose; 0 when every pass ran through LDS. */
+size_t eithot_last_peak_extra_bytes(void);
+
+/* Message of the calling thread's last failure ("" if none). */
+const char* eithot_last_error(void);
+
+/* Frees the scratch arenas and cached plans of every device. */
+void eithot_release(void);
+
+/* ---- planner / estimator (host only, no GPU needed) ------------------------ */
+
+/* The Nd_estimation planner for this build's kernels: sub-tensor count, the
+ * extra-memory ratio the plan needs at most, and its scratch bytes.
+ * Replaces the dry-run inplace::transpose of
+ * Nd_memory_estimation/include/transpose.h:8 as used by
+ * Nd_memory_estimation/src/test_inplace.cpp:18-147. */
+int eithot_estimate(const int64_t* dims, const int* perm, int order, size_t elem_bytes,
+                    double alpha, int* n_sub_out, double* alpha_out, size_t* extra_bytes_out);
+
+/* The reference estimator's own policy and size model evaluated on a CUDA
+ * device profile (shared memory per block, SM count, max threads per SM,
+ * occupancy of its padding kernel), reproducing what Nd_estimation prints
+ * (Nd_memory_estimation/src/test_inplace.cpp:18-147).  Outputs: init and
+ * final extra bytes, nb, alpha written to proper_input.txt, and flags
+ * (bit0: "new ratio" printed, bit1: "Final Extra Memory" printed). */
+typedef struct {
+  int64_t smem_per_block;
+  int sms;
+  int max_threads_per_sm;
+  int padding_blocks_per_sm;
+} eithot_ref_profile;
+
+int eithot_estimate_reference(const int64_t* dims, const int* perm, int order, size_t elem_bytes,
+                              double alpha, const eithot_ref_profile* prof, double* init_bytes,
+                              double* new_ratio, double* final_bytes, double* final_ratio,
+                              int* nb, double* alpha_out, int* flags);
+
+/* Decomposer rounds (permutation_util.cpp, Nd_transpose.cpp:146-250):
+ * reorder 0 = mix, 1 = pre-order, 2 = post-order.  Per stage: rank, a 1-based
+ * low-order perm (4 ints) and merged dims (4 int64).  Returns the stage count. */
+int eithot_decompose(const int64_t* dims, const int* perm, int order, int reorder,
+                     int* stage_rank, int* stage_perm, int64_t* stage_dims, int max_stages);
+
+/* One device pass of a plan, for introspection and tests. */
+typedef struct {
+  int kind;      /* 1 tile, 2 column, 3 row */
+  int map;       /* 0 none, 1 rot, 2 rotinv, 3 colsh, 4 colshinv, 5 rowg, 6 rowd */
+  int staged;    /* 1 when executed through scratch */
+  int stage;     /* decomposer stage, -1 for partition/join */
+  uint64_t base; /* byte offset */
+  int64_t B, R, C, U, us, W;
+  int64_t m, n, c, a, b, k, binv;
+} eithot_op_desc;
+
+/* Fills up to max_ops pass descriptors of the plan for the gfx950 profile
+ * (lds_max bytes of LDS per workgroup; <= 0 selects 160 KiB) and returns
+ * the pass count; *scratch_bytes receives the plan's scratch need. */
+int eithot_plan_describe(const int64_t* dims, const int* perm, int order, size_t elem_bytes,
+                         int n_sub, double alpha, int64_t lds_max, eithot_op_desc* ops,
+                         int max_ops, int64_t* scratch_bytes);
+
+/* ---- measurement helpers ----------------------------------------------------- */
+
+/* When enabled, the next transposes record per-pass GPU time (HIP events on
+ * the launching stream); eithot_last_pass_times copies up to max of them (ms). */
+void eithot_set_pass_timing(int enable);
+int eithot_last_pass_times(float* ms, int max);
+
+/* Device-side pattern fill and check, for tensors too large to copy to the
+ * host: fills v(seed, first+i) (mode 0: v = i, mode 1: 64-bit mix of seed
+ * and i) and counts output elements whose value differs from the permuted
+ * pattern. */
+int eithot_fill_pattern(void* dev, uint64_t n, size_t elem_bytes, uint64_t seed, int mode,
+                        uint64_t first, void* stream);
+int eithot_verify_pattern(const void* dev, const int64_t* dims, const int* perm, int order,
+                          size_t elem_bytes, uint64_t seed, int mode,
+                          unsigned long long* mismatches);
+
+/* ---- batched transposition (many small tensors per launch) -------------------- */
+
+typedef struct eithot_batch eithot_batch;
+
+eithot_batch* eithot_batch_create(void);
+/* Adds one tensor (device pointer, 1-based perm).  Returns its index. */
+int eithot_batch_add(eithot_batch* b, void* dev, const int64_t* dims, const int* perm, int order,
+                     size_t elem_bytes);
+/* Plans every tensor, groups passes by stage index into persistent launches,
+ * runs the autotuner (tune != 0) and captures the launches in a HIP graph. */
+int eithot_batch_build(eithot_batch* b, double alpha, int tune);
+/* Replays the batch on `stream` (NULL: null stream) and waits when sync. */
+int eithot_batch_run(eithot_batch* b, void* stream, int sync);
+/* Launch count per replay and the batch's scratch bytes. */
+int eithot_batch_info(const eithot_batch* b, int* launches, int64_t* scratch_bytes,
+                      int* graph_captured);
+void eithot_batch_destroy(eithot_batch* b);
+
+/* ---- serving (continuous batching of transpose requests) ----------------------- */
+
+typedef struct eithot_server eithot_server;
+
+/* scratch_pool_bytes is the shared extra-memory budget all in-flight requests
+ * are admitted against; max_inflight caps concurrent requests; nstreams is
+ * the number of device streams requests are spread over. */
+eithot_server* eithot_server_create(int64_t scratch_pool_bytes, int max_inflight, int nstreams);
+/* Enqueues a request on resident tensor `dev`; returns a request id >= 0. */
+int64_t eithot_server_submit(eithot_server* s, void* dev, const int64_t* dims, const int* perm,
+                             int order, size_t elem_bytes);
+/* Admits queued requests while slots and scratch allow and retires finished
+ * ones; returns the number of requests completed by this call.  With wait
+ * != 0 it blocks until at least one in-flight request completes. */
+int eithot_server_poll(eithot_server* s, int wait);
+/* Status of a request: 0 queued, 1 running, 2 done, <0 error code.
+ * On done, *latency_ms holds submit-to-completion time (host clock). */
+int eithot_server_status(eithot_server* s, int64_t id, double* latency_ms);
+/* Blocks until every submitted request has completed. */
+int eithot_server_drain(eithot_server* s);
+void eithot_server_destroy(eithot_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EITHOT_H */
