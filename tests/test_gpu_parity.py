@@ -1,0 +1,59 @@
+"""GPU parity: the HIP path through the C-ABI equals the oracle, bit for bit."""
+import itertools
+
+import numpy as np
+import pytest
+
+from gpu_util import assert_bitexact, run_case
+
+pytestmark = pytest.mark.gpu
+
+
+def test_config1_known_answer(torch_cuda):
+    # (2,4096,1024,2) perm 1 3 2 4, v(i) = i: the reference's own check
+    got, want, peak = run_case(torch_cuda, [2, 4096, 1024, 2], [1, 3, 2, 4], 4, mode=0)
+    assert_bitexact(got, want, "config 1")
+    assert peak <= 0.05 * got.nbytes
+
+
+@pytest.mark.parametrize("elem", [1, 2, 4, 8])
+def test_all_perms_rank4_small(torch_cuda, elem):
+    dims = [3, 5, 4, 7]
+    for perm in itertools.permutations([1, 2, 3, 4]):
+        got, want, _ = run_case(torch_cuda, dims, list(perm), elem, seed=2)
+        assert_bitexact(got, want, f"perm {perm}")
+
+
+def test_catanzaro_lds_shapes(torch_cuda):
+    # column/row passes on LDS: gcd > 1 (rotate present) and gcd == 1, C2R and R2C
+    for dims in ([384, 256], [256, 384], [1000, 999], [999, 1000], [2, 640, 480, 2],
+                 [4, 96, 72, 3]):
+        r = len(dims)
+        perm = [2, 1] if r == 2 else [1, 3, 2, 4]
+        for elem in (1, 4, 8):
+            got, want, _ = run_case(torch_cuda, dims, perm, elem, seed=3)
+            assert_bitexact(got, want, f"{dims} elem {elem}")
+
+
+def test_staged_passes(torch_cuda):
+    # rows longer than LDS force scratch-staged passes (alpha allows them)
+    for dims, perm in (([3, 100003], [2, 1]), ([100003, 3], [2, 1]), ([7, 65537], [2, 1]),
+                       ([2, 3001, 1801, 2], [1, 3, 2, 4])):
+        got, want, peak = run_case(torch_cuda, dims, perm, 4, seed=4, alpha=1.0)
+        assert_bitexact(got, want, f"{dims}")
+        assert peak <= 1.0 * got.nbytes
+
+
+def test_random_high_order(torch_cuda):
+    rng = np.random.default_rng(123)
+    for _ in range(60):
+        r = int(rng.integers(2, 7))
+        dims = [int(x) for x in np.exp(rng.uniform(0, np.log(400), size=r)).astype(int) + 1]
+        while np.prod(dims) > 2_000_000:
+            dims[int(np.argmax(dims))] //= 2
+        perm = [int(p) for p in rng.permutation(r) + 1]
+        elem = int(rng.choice([1, 2, 4, 8]))
+        got, want, peak = run_case(torch_cuda, dims, perm, elem, seed=int(rng.integers(1 << 30)),
+                                   alpha=0.5)
+        assert_bitexact(got, want, f"{dims} {perm} elem {elem}")
+        assert peak <= 0.5 * got.nbytes

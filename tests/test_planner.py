@@ -1,0 +1,159 @@
+"""Host-side product checks (no GPU): decomposer parity with the oracle and
+the reference's hand-traced plans, plan lowering executed by the oracle,
+estimator golden vectors, error handling of the C-ABI."""
+import glob
+import json
+import os
+
+import numpy as np
+import pytest
+
+import eithot_amd as E
+from oracle import pyoracle as O
+from plan_sim import simulate
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+CORPUS = "/root/reference/Benchmark/test_cases"
+
+
+def test_golden_plans():
+    for case in json.load(open(os.path.join(GOLD, "plans.json")))["cases"]:
+        got = E.decompose(case["dims"], case["perm"], case["reorder"])
+        assert got == [(tuple(p), tuple(d)) for p, d in case["stages"]], case
+
+
+def test_decomposer_matches_oracle_random():
+    rng = np.random.default_rng(3)
+    for _ in range(2000):
+        r = int(rng.integers(1, 9))
+        dims = [int(x) for x in rng.integers(1, 70, size=r)]
+        perm1 = [int(p) + 1 for p in rng.permutation(r)]
+        for reorder in (0, 1, 2):
+            assert E.decompose(dims, perm1, reorder) == O.decompose(dims, perm1, reorder)
+
+
+@pytest.mark.skipif(not os.path.isdir(CORPUS), reason="reference corpus not present")
+def test_decomposer_matches_oracle_on_reference_corpus():
+    n = 0
+    for f in sorted(glob.glob(os.path.join(CORPUS, "**", "*.txt"), recursive=True)):
+        for line in open(f):
+            x = line.split()
+            if not x:
+                continue
+            r = (len(x) - 3) // 2
+            dims, perm1 = [int(v) for v in x[:r]], [int(v) for v in x[r:2 * r]]
+            for reorder in (0, 1, 2):
+                assert E.decompose(dims, perm1, reorder) == O.decompose(dims, perm1, reorder), line
+                n += 1
+    assert n > 10000
+
+
+def _random_case(rng, maxvol=300000):
+    r = int(rng.integers(1, 7))
+    dims = [int(x) for x in np.exp(rng.uniform(0, np.log(300), size=r)).astype(int) + 1]
+    while np.prod(dims) > maxvol:
+        dims[int(np.argmax(dims))] //= 2
+    perm1 = [int(p) + 1 for p in rng.permutation(r)]
+    return dims, perm1
+
+
+@pytest.mark.parametrize("lds", [0, 2048, 16384])
+def test_plans_execute_correctly_on_oracle(lds):
+    # small LDS budgets force column/row Catanzaro passes and staged passes
+    rng = np.random.default_rng(lds + 1)
+    seen = set()
+    for _ in range(150):
+        dims, perm1 = _random_case(rng)
+        elem = int(rng.choice([1, 2, 4, 8]))
+        vol = int(np.prod(dims))
+        a = O.fill(vol, elem, 5, 1)
+        want = O.permute(a, dims, [p - 1 for p in perm1], elem)
+        ops, scratch = E.plan_describe(dims, perm1, elem, 1, 1.0, lds)
+        seen |= {(o["kind"], o["map"], o["staged"]) for o in ops}
+        b = a.copy()
+        simulate(b, ops)
+        assert np.array_equal(b, want), (dims, perm1, elem)
+        assert scratch <= max(vol * elem, 1)
+    if lds == 2048:
+        assert (3, 5, 1) in seen or (3, 6, 1) in seen  # some staged row pass was exercised
+
+
+def test_partition_plan_executes_correctly():
+    for dims, perm1, nsub in (([152, 150], [2, 1], 4), ([6, 400, 3], [3, 2, 1], 8),
+                              ([96, 40, 5], [2, 3, 1], 4)):
+        vol = int(np.prod(dims))
+        ops, _ = E.plan_describe(dims, perm1, 4, nsub, 1.0)
+        assert any(o["stage"] == -1 for o in ops), "partition/join passes expected"
+        a = O.fill(vol, 4, 6, 1)
+        b = a.copy()
+        simulate(b, ops)
+        assert np.array_equal(b, O.permute(a, dims, [p - 1 for p in perm1], 4))
+
+
+def test_config_plans():
+    # config 1: three LDS Catanzaro passes, no scratch
+    ops, scratch = E.plan_describe([2, 4096, 1024, 2], [1, 3, 2, 4], 4, 1, 0.05)
+    assert [(o["kind"], o["map"], o["staged"]) for o in ops] == [(2, 1, 0), (3, 5, 0), (2, 3, 0)]
+    assert scratch == 0
+    # config 2: every pass through LDS
+    ops, scratch = E.plan_describe([32, 48, 64, 40, 36, 8], [6, 4, 2, 5, 1, 3], 4, 1, 0.05)
+    assert scratch == 0 and all(not o["staged"] for o in ops)
+    # config 3: scratch within 5 % of 171.8 GB
+    ops, scratch = E.plan_describe([2, 163840, 65536, 2], [1, 3, 2, 4], 4, 1, 0.05)
+    assert 0 < scratch <= 0.05 * 2 * 163840 * 65536 * 2 * 4
+
+
+def test_estimator_golden_exp2():
+    doc = json.load(open(os.path.join(GOLD, "estimator_exp2.json")))
+    for c in doc["cases"]:
+        e = E.estimate_reference(c["dims"], c["perm"], 4, 0.01)
+        assert round(e["init_bytes"] / 1e6, 5) == pytest.approx(c["init_mb"], abs=1e-5), c
+        assert e["nb"] == c["nb"], c
+        assert e["alpha"] == pytest.approx(c["alpha"]), c
+        assert e["printed_new_ratio"] == (e["init_bytes"] > 0)
+        if e["printed_new_ratio"]:
+            assert e["new_ratio"] == pytest.approx(doc["printed_new_ratio"])
+    e = E.estimate_reference([7248, 7248], [2, 1], 4, 0.01)
+    assert e["decomposed"]
+    assert round(e["final_bytes"] / 1e6, 5) == doc["case1_final"]["final_mb"]
+    assert e["final_ratio"] == pytest.approx(doc["case1_final"]["ratio"])
+
+
+def test_native_estimate_within_alpha():
+    rng = np.random.default_rng(9)
+    for _ in range(200):
+        dims, perm1 = _random_case(rng, 10 ** 7)
+        for alpha in (0.01, 0.05, 0.5):
+            try:
+                n, need, extra = E.estimate(dims, perm1, 4, alpha)
+            except E.EithotError as err:
+                assert err.code == -3  # the plan cannot fit alpha
+                continue
+            vol = int(np.prod(dims))
+            assert extra <= max(alpha * vol * 4, 1) + 1
+            assert need == pytest.approx(extra / (vol * 4))
+
+
+@pytest.mark.parametrize("dims,perm,elem,alpha,code", [
+    ([4, 4], [1, 1], 4, 0.05, -1),      # not a permutation
+    ([4, 4], [0, 1], 4, 0.05, -1),      # 0 is not 1-based
+    ([4, 4, 4], [1, 2, 4], 4, 0.05, -1),  # out of range
+    ([0, 4], [2, 1], 4, 0.05, -1),      # empty dim
+    ([4, -3], [2, 1], 4, 0.05, -1),     # negative dim
+    ([4, 4], [2, 1], 3, 0.05, -2),      # unsupported element size
+    ([4, 4], [2, 1], 16, 0.05, -2),
+    ([4, 4], [2, 1], 4, 0.0, -1),       # alpha must be positive
+])
+def test_transpose_rejects_invalid_input_before_touching_the_device(dims, perm, elem, alpha, code):
+    import ctypes
+    fake = ctypes.c_void_p(1 << 20)  # never dereferenced: validation fails first
+    rc = E.lib().eithot_transpose(fake, E.i64(dims), E.i32(perm), len(dims), elem, 1, alpha)
+    assert rc == code
+    assert E.lib().eithot_last_error()
+
+
+def test_reference_signature_rejects_bad_rank():
+    import ctypes
+    rc = E.lib().eithot_inplace_transpose(ctypes.c_void_p(1 << 20), 0, 0, E.i32([1]), E.i32([0]),
+                                          4, 1, 0.05)
+    assert rc == -1
