@@ -4,14 +4,16 @@
 //   OP_TILE  one B-matrix (R x C units), written back transposed (C x R)
 //   OP_COL   one column panel: W adjacent columns x all R rows of the grid
 //   OP_ROW   one row of C units
-// optionally restricted to one `us`-byte slice of every unit (units larger
-// than a footprint budget are cut into slices that are permuted alike).
+// optionally restricted to one `us`-byte slice of every unit (a "cell";
+// units larger than a footprint budget are cut into slices permuted alike).
 //
-// LDS kernel: a workgroup loads K footprints into LDS with coalesced reads,
-// barriers, and writes them back in gather order (dest unit q <- LDS unit
-// src(q)), so reads and writes are both full-line HBM transactions.
-// Staged kernels: footprints larger than LDS are copied to a scratch buffer
-// (stage-out) and gathered back (stage-in); two launches on one stream.
+// LDS kernels: a workgroup loads K footprints into LDS (coalesced 16-byte
+// loads, UNR of them in flight per thread), barriers, and writes them back
+// in gather order with 16-byte stores that pack several small cells.  Tile
+// rows in LDS are padded so that column reads hit distinct banks.
+// Staged kernels (footprints larger than LDS): the footprints are copied to
+// a scratch buffer and written back through the same index maps; tiles go
+// through an LDS-tiled out-of-place transpose from scratch.
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -22,21 +24,15 @@
 namespace eithot {
 namespace {
 
+constexpr int UNR = 8;  // global loads in flight per thread
+
 struct alignas(16) W16 {
   uint32_t v[4];
 };
 
-template <typename Idx>
-struct PassArgs {
-  char* data;
-  char* scratch;
-  uint64_t base;  // byte offset of footprint origin (batch 0)
-  uint64_t U, us, RCU, CU;
-  Idx R, C, W, S, NU, wu, K, nfp, P, per_fp;
-  uint64_t fp0, G;  // staged: first footprint and count of this group
-  int contiguous;   // footprint group is one contiguous byte range (S == 1, not COL)
-  typename FDsel<Idx>::type fS, fR, fW, fP, fwu, fper;
-  GridMaps<Idx> g;
+template <int BYTES>
+struct alignas(BYTES) Vec {
+  unsigned char b[BYTES];
 };
 
 __device__ __forceinline__ uint32_t xcd_logical(uint32_t bid, uint32_t n, int remap) {
@@ -46,22 +42,66 @@ __device__ __forceinline__ uint32_t xcd_logical(uint32_t bid, uint32_t n, int re
   return (bid % 8) * per + bid / 8;
 }
 
+// ---------------------------------------------------------------------------
+// common arguments
+// ---------------------------------------------------------------------------
+// 16-byte LDS store at a 4-byte aligned offset (padded rows may break
+// 16-byte alignment)
+__device__ __forceinline__ void lds_put16(unsigned char* p, const W16& v) {
+  const uintptr_t a = (uintptr_t)p;
+  if ((a & 15) == 0) {
+    *reinterpret_cast<W16*>(p) = v;
+  } else if ((a & 7) == 0) {
+    reinterpret_cast<uint64_t*>(p)[0] = (uint64_t)v.v[0] | ((uint64_t)v.v[1] << 32);
+    reinterpret_cast<uint64_t*>(p)[1] = (uint64_t)v.v[2] | ((uint64_t)v.v[3] << 32);
+  } else {
+    uint32_t* q = reinterpret_cast<uint32_t*>(p);
+    q[0] = v.v[0];
+    q[1] = v.v[1];
+    q[2] = v.v[2];
+    q[3] = v.v[3];
+  }
+}
+
+template <typename Idx>
+struct Args {
+  char* data;
+  const char* src;   // staged tile: scratch holding the footprints
+  char* scratch;
+  uint64_t base;     // byte offset of batch 0
+  uint64_t U, us, RCU, CU;
+  Idx R, C, W, S, K, nfp, P, wu;
+  Idx rs;            // LDS bytes per footprint row (padded)
+  Idx fpl;           // LDS bytes per footprint
+  Idx cpr;           // 16-byte chunks per footprint row (0: load word by word)
+  Idx ld_per;        // load-loop items per footprint
+  Idx st_per;        // store-loop items per footprint
+  Idx vpr;           // store vectors per output row (vector stores)
+  Idx cps;           // 16-byte chunks per cell (0: cells are not whole chunks)
+  int cps_log2;      // log2(cps) when cps is a power of two, else -1
+  uint64_t fp0, G;   // staged: first footprint and count of the group
+  int contiguous;    // S == 1 and the K footprints are one contiguous byte range
+  typename FDsel<Idx>::type fS, fR, fC, fW, fP, fwu, fcpr, fld, fst, fvpr, fcps;
+  GridMaps<Idx> g;
+};
+
+// byte address (offset from data) of cell 0 of footprint f, and its grid role
 template <typename Idx, int KIND>
 struct Foot {
-  uint64_t addr;  // byte address (offset from data) of unit 0 of this footprint slice
-  Idx row;        // OP_ROW: row within the grid; OP_COL: first column of the panel
-  Idx weff;       // OP_COL: columns present in the panel
+  uint64_t addr;
+  Idx row;   // OP_ROW: row within the grid; OP_COL: first column of the panel
+  Idx weff;  // OP_COL: columns present in the panel
 };
 
 template <typename Idx, int KIND>
-__device__ __forceinline__ Foot<Idx, KIND> decode(const PassArgs<Idx>& a, Idx f) {
+__device__ __forceinline__ Foot<Idx, KIND> decode(const Args<Idx>& a, Idx f) {
   Foot<Idx, KIND> r;
-  const Idx o = a.fS.div(f);
+  const Idx o = a.S == 1 ? f : a.fS.div(f);
   const Idx s = f - o * a.S;
+  r.row = 0;
+  r.weff = 0;
   if (KIND == OP_TILE) {
     r.addr = a.base + (uint64_t)o * a.RCU + (uint64_t)s * a.us;
-    r.row = 0;
-    r.weff = 0;
   } else if (KIND == OP_COL) {
     const Idx b = a.fP.div(o);
     const Idx p = o - b * a.P;
@@ -72,138 +112,518 @@ __device__ __forceinline__ Foot<Idx, KIND> decode(const PassArgs<Idx>& a, Idx f)
   } else {
     r.addr = a.base + (uint64_t)o * a.CU + (uint64_t)s * a.us;
     r.row = a.fR.mod(o);
-    r.weff = 0;
   }
   return r;
 }
 
-// byte offset of footprint unit q (LDS order == memory order of the footprint)
-template <typename Idx, int KIND>
-__device__ __forceinline__ uint64_t unit_off(const PassArgs<Idx>& a, Idx q, Idx* col) {
-  if (KIND == OP_COL) {
-    const Idx i = a.fW.div(q);
-    const Idx jj = q - i * a.W;
-    *col = jj;
-    return (uint64_t)i * a.CU + (uint64_t)jj * a.U;
-  }
-  *col = 0;
-  return (uint64_t)q * a.U;
+// source row of the column pass / source cell of the row pass
+template <typename Idx, int MAP>
+__device__ __forceinline__ Idx col_src(const Args<Idx>& a, Idx i, Idx j) {
+  if (MAP == MAP_ROT) return a.g.rot(i, j);
+  if (MAP == MAP_ROTINV) return a.g.rotinv(i, j);
+  if (MAP == MAP_COLSH) return a.g.colsh(i, j);
+  return a.g.colshinv(i, j);
+}
+template <typename Idx, int MAP>
+__device__ __forceinline__ Idx row_src(const Args<Idx>& a, Idx i, Idx j) {
+  if (MAP == MAP_ROWG) return a.g.rowg(i, j);
+  return a.g.rowd(i, j);
 }
 
-// footprint unit that holds the data for destination unit q
-template <typename Idx, int KIND, int MAP>
-__device__ __forceinline__ Idx src_unit(const PassArgs<Idx>& a, const Foot<Idx, KIND>& f, Idx q) {
-  if (KIND == OP_TILE) {  // q = j*R + i in the transposed C x R output
-    const Idx j = a.fR.div(q);
-    const Idx i = q - j * a.R;
-    return i * a.C + j;
-  } else if (KIND == OP_COL) {
-    const Idx i = a.fW.div(q);
-    const Idx jj = q - i * a.W;
-    const Idx j = f.row + jj;
-    Idx s;
-    if (MAP == MAP_ROT) s = a.g.rot(i, j);
-    else if (MAP == MAP_ROTINV) s = a.g.rotinv(i, j);
-    else if (MAP == MAP_COLSH) s = a.g.colsh(i, j);
-    else s = a.g.colshinv(i, j);
-    return s * a.W + jj;
-  } else {
-    if (MAP == MAP_ROWG) return a.g.rowg(f.row, q);
-    return a.g.rowd(f.row, q);
-  }
-}
-
-template <typename Word, typename Idx, int KIND, int MAP>
-__global__ __launch_bounds__(256) void k_lds(PassArgs<Idx> a, int remap) {
+// ---------------------------------------------------------------------------
+// tile pass: K matrices (or matrix slices) per workgroup
+// LDS: footprint k row i at k*fpl + i*rs (rs padded), cell j at +j*us
+// ---------------------------------------------------------------------------
+template <typename Word, typename Idx, int VEC>
+__global__ __launch_bounds__(512) void k_tile(Args<Idx> a, int remap) {
   extern __shared__ __align__(16) unsigned char lds[];
-  Word* L = reinterpret_cast<Word*>(lds);
   const Idx wg = (Idx)xcd_logical(blockIdx.x, gridDim.x, remap);
   const Idx f0 = wg * a.K;
   const Idx nf = (a.nfp - f0) < a.K ? (a.nfp - f0) : a.K;
-  const Idx total = nf * a.per_fp;
-  const Idx wu = a.wu;
   const Idx tid = threadIdx.x, nt = blockDim.x;
+  constexpr Idx w = sizeof(Word);
 
-  if (a.contiguous) {
-    // the K footprints form one contiguous byte range: plain coalesced copy
-    const Foot<Idx, KIND> ff = decode<Idx, KIND>(a, f0);
-    const char* src = a.data + ff.addr;
-    const uint64_t nbytes = (uint64_t)total * sizeof(Word);
-    if ((((uintptr_t)src) & 15) == 0 && (nbytes & 15) == 0) {
-      const W16* s16 = reinterpret_cast<const W16*>(src);
-      W16* l16 = reinterpret_cast<W16*>(lds);
-      const Idx n16 = (Idx)(nbytes / 16);
-      for (Idx x = tid; x < n16; x += nt) l16[x] = s16[x];
-    } else {
-      const Word* sw = reinterpret_cast<const Word*>(src);
-      for (Idx x = tid; x < total; x += nt) L[x] = sw[x];
+  if (a.cpr) {
+    // nf whole contiguous matrices: rows of C*us bytes in 16-byte chunks;
+    // row r of the group lands at r*rs (fpl == R*rs)
+    const W16* g = reinterpret_cast<const W16*>(a.data + a.base + (uint64_t)f0 * a.RCU);
+    const Idx n = nf * a.R * a.cpr;
+    for (Idx t0 = tid; t0 < n; t0 += nt * UNR) {
+      W16 v[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const Idx t = t0 + (Idx)u * nt;
+        if (t < n) v[u] = g[t];
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const Idx t = t0 + (Idx)u * nt;
+        if (t < n) {
+          const Idx r = a.fcpr.div(t);
+          lds_put16(lds + r * a.rs + (t - r * a.cpr) * 16, v[u]);
+        }
+      }
+    }
+  } else if (a.cps) {
+    // cells of whole 16-byte chunks (sliced units): chunk t -> footprint k,
+    // cell q = i*C + j, chunk c; global cells are U bytes apart
+    const Idx per = a.R * a.C * a.cps;
+    const Idx n = nf * per;
+    const Foot<Idx, OP_TILE> f1 = decode<Idx, OP_TILE>(a, f0);
+    for (Idx t0 = tid; t0 < n; t0 += nt * UNR) {
+      W16 v[UNR];
+      Idx off[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const Idx t = t0 + (Idx)u * nt;
+        if (t < n) {
+          Idx k = 0, rem = t;
+          uint64_t addr = f1.addr;
+          if (nf > 1) {
+            k = a.fld.div(t);
+            rem = t - k * per;
+            addr = decode<Idx, OP_TILE>(a, f0 + k).addr;
+          }
+          const Idx q = a.cps_log2 >= 0 ? rem >> a.cps_log2 : a.fcps.div(rem);
+          const Idx c = rem - q * a.cps;
+          const Idx i = a.fC.div(q);
+          const Idx j = q - i * a.C;
+          v[u] = *reinterpret_cast<const W16*>(a.data + addr + (uint64_t)q * a.U + c * 16);
+          off[u] = k * a.fpl + i * a.rs + j * (Idx)a.us + c * 16;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const Idx t = t0 + (Idx)u * nt;
+        if (t < n) *reinterpret_cast<W16*>(lds + off[u]) = v[u];
+      }
     }
   } else {
-    for (Idx idx = tid; idx < total; idx += nt) {
-      const Idx fl = a.K == 1 ? 0 : a.fper.div(idx);
-      const Idx rem = idx - fl * a.per_fp;
-      const Idx q = wu == 1 ? rem : a.fwu.div(rem);
-      const Idx x = rem - q * wu;
-      const Foot<Idx, KIND> f = decode<Idx, KIND>(a, f0 + fl);
-      Idx col;
-      const uint64_t off = unit_off<Idx, KIND>(a, q, &col);
-      if (KIND == OP_COL && col >= f.weff) continue;
-      L[idx] = *reinterpret_cast<const Word*>(a.data + f.addr + off + (uint64_t)x * sizeof(Word));
+    // word granular: footprint k, cell q = i*C + j, word x
+    const Idx n = nf * a.ld_per;
+    for (Idx t0 = tid; t0 < n; t0 += nt * UNR) {
+      Word v[UNR];
+      Idx off[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const Idx t = t0 + (Idx)u * nt;
+        if (t < n) {
+          const Idx k = a.fld.div(t);
+          const Idx rem = t - k * a.ld_per;
+          const Idx q = a.wu == 1 ? rem : a.fwu.div(rem);
+          const Idx x = rem - q * a.wu;
+          const Idx i = a.fC.div(q);
+          const Idx j = q - i * a.C;
+          const Foot<Idx, OP_TILE> f = decode<Idx, OP_TILE>(a, f0 + k);
+          v[u] = *reinterpret_cast<const Word*>(a.data + f.addr + (uint64_t)q * a.U +
+                                                (uint64_t)x * w);
+          off[u] = k * a.fpl + i * a.rs + j * (Idx)a.us + x * w;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const Idx t = t0 + (Idx)u * nt;
+        if (t < n) *reinterpret_cast<Word*>(lds + off[u]) = v[u];
+      }
     }
   }
   __syncthreads();
-  for (Idx idx = tid; idx < total; idx += nt) {
-    const Idx fl = a.K == 1 ? 0 : a.fper.div(idx);
-    const Idx rem = idx - fl * a.per_fp;
-    const Idx q = wu == 1 ? rem : a.fwu.div(rem);
-    const Idx x = rem - q * wu;
-    const Foot<Idx, KIND> f = decode<Idx, KIND>(a, f0 + fl);
-    Idx col;
-    const uint64_t off = unit_off<Idx, KIND>(a, q, &col);
-    if (KIND == OP_COL && col >= f.weff) continue;
-    const Idx s = src_unit<Idx, KIND, MAP>(a, f, q);
-    *reinterpret_cast<Word*>(a.data + f.addr + off + (uint64_t)x * sizeof(Word)) =
-        L[(uint64_t)fl * a.per_fp + (uint64_t)s * wu + x];
+
+  if (VEC > 1) {
+    // S == 1, one word per cell: VEC cells (consecutive i of output row j)
+    // per store; output row j of matrix k starts at (j*R)*us
+    const Idx n = nf * a.st_per;
+    char* gbase = a.data + a.base + (uint64_t)f0 * a.RCU;
+    for (Idx t = tid; t < n; t += nt) {
+      const Idx k = a.fst.div(t);
+      const Idx rem = t - k * a.st_per;
+      const Idx j = a.fvpr.div(rem);
+      const Idx i0 = (rem - j * a.vpr) * VEC;
+      Word tmp[VEC];
+      const unsigned char* l = lds + k * a.fpl + i0 * a.rs + j * w;
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) tmp[v] = *reinterpret_cast<const Word*>(l + v * a.rs);
+      *reinterpret_cast<Vec<VEC * sizeof(Word)>*>(gbase + (uint64_t)k * a.RCU +
+                                                  ((uint64_t)j * a.R + i0) * w) =
+          *reinterpret_cast<const Vec<VEC * sizeof(Word)>*>(tmp);
+    }
+  } else if (a.cps) {
+    const Idx per = a.R * a.C * a.cps;
+    const Idx n = nf * per;
+    const Foot<Idx, OP_TILE> f1 = decode<Idx, OP_TILE>(a, f0);
+    for (Idx t = tid; t < n; t += nt) {
+      Idx k = 0, rem = t;
+      uint64_t addr = f1.addr;
+      if (nf > 1) {
+        k = a.fld.div(t);
+        rem = t - k * per;
+        addr = decode<Idx, OP_TILE>(a, f0 + k).addr;
+      }
+      const Idx q = a.cps_log2 >= 0 ? rem >> a.cps_log2 : a.fcps.div(rem);  // q = j*R + i
+      const Idx c = rem - q * a.cps;
+      const Idx j = a.fR.div(q);
+      const Idx i = q - j * a.R;
+      *reinterpret_cast<W16*>(a.data + addr + (uint64_t)q * a.U + c * 16) =
+          *reinterpret_cast<const W16*>(lds + k * a.fpl + i * a.rs + j * (Idx)a.us + c * 16);
+    }
+  } else {
+    const Idx n = nf * a.st_per;
+    for (Idx t = tid; t < n; t += nt) {
+      const Idx k = a.fst.div(t);
+      const Idx rem = t - k * a.st_per;
+      const Idx q = a.wu == 1 ? rem : a.fwu.div(rem);  // output cell q = j*R + i
+      const Idx x = rem - q * a.wu;
+      const Idx j = a.fR.div(q);
+      const Idx i = q - j * a.R;
+      const Foot<Idx, OP_TILE> f = decode<Idx, OP_TILE>(a, f0 + k);
+      *reinterpret_cast<Word*>(a.data + f.addr + (uint64_t)q * a.U + (uint64_t)x * w) =
+          *reinterpret_cast<const Word*>(lds + k * a.fpl + i * a.rs + j * (Idx)a.us + x * w);
+    }
   }
 }
 
-// staged pass, part 1: footprints [fp0, fp0+G) -> scratch (footprint-major)
+// ---------------------------------------------------------------------------
+// column pass: panels of W columns x R rows; LDS row i of footprint k holds
+// the panel's segment (W*us bytes) at k*fpl + i*rs
+// ---------------------------------------------------------------------------
+template <typename Word, typename Idx, int MAP, int VEC>
+__global__ __launch_bounds__(512) void k_col(Args<Idx> a, int remap) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const Idx wg = (Idx)xcd_logical(blockIdx.x, gridDim.x, remap);
+  const Idx f0 = wg * a.K;
+  const Idx nf = (a.nfp - f0) < a.K ? (a.nfp - f0) : a.K;
+  const Idx tid = threadIdx.x, nt = blockDim.x;
+  constexpr Idx w = sizeof(Word);
+
+  {
+    // a.cpr != 0: every panel is full and its segment is whole 16-byte chunks
+    const Idx n = nf * a.ld_per;
+    for (Idx t0 = tid; t0 < n; t0 += nt * UNR) {
+      W16 v16[UNR];
+      Word vw[UNR];
+      Idx off[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const Idx t = t0 + (Idx)u * nt;
+        off[u] = ~(Idx)0;
+        if (t < n) {
+          const Idx k = a.fld.div(t);
+          const Idx rem = t - k * a.ld_per;
+          const Foot<Idx, OP_COL> f = decode<Idx, OP_COL>(a, f0 + k);
+          if (a.cpr) {
+            const Idx i = a.fcpr.div(rem);
+            const Idx c = rem - i * a.cpr;
+            v16[u] = *reinterpret_cast<const W16*>(a.data + f.addr + (uint64_t)i * a.CU + c * 16);
+            off[u] = k * a.fpl + i * a.rs + c * 16;
+          } else {
+            const Idx q = a.wu == 1 ? rem : a.fwu.div(rem);
+            const Idx x = rem - q * a.wu;
+            const Idx i = a.fW.div(q);
+            const Idx jj = q - i * a.W;
+            if (jj < f.weff) {
+              vw[u] = *reinterpret_cast<const Word*>(a.data + f.addr + (uint64_t)i * a.CU +
+                                                     (uint64_t)jj * a.U + x * w);
+              off[u] = k * a.fpl + i * a.rs + jj * (Idx)a.us + x * w;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        if (off[u] == ~(Idx)0) continue;
+        if (a.cpr) *reinterpret_cast<W16*>(lds + off[u]) = v16[u];
+        else *reinterpret_cast<Word*>(lds + off[u]) = vw[u];
+      }
+    }
+  }
+  __syncthreads();
+
+  if (VEC > 1) {
+    // full panels, one word per cell: VEC adjacent columns of row i per store
+    const Idx n = nf * a.st_per;
+    for (Idx t = tid; t < n; t += nt) {
+      const Idx k = a.fst.div(t);
+      const Idx rem = t - k * a.st_per;
+      const Idx i = a.fvpr.div(rem);
+      const Idx jj0 = (rem - i * a.vpr) * VEC;
+      const Foot<Idx, OP_COL> f = decode<Idx, OP_COL>(a, f0 + k);
+      Word tmp[VEC];
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        const Idx s = col_src<Idx, MAP>(a, i, f.row + jj0 + v);
+        tmp[v] = *reinterpret_cast<const Word*>(lds + k * a.fpl + s * a.rs + (jj0 + v) * w);
+      }
+      *reinterpret_cast<Vec<VEC * sizeof(Word)>*>(a.data + f.addr + (uint64_t)i * a.CU +
+                                                  (uint64_t)jj0 * w) =
+          *reinterpret_cast<const Vec<VEC * sizeof(Word)>*>(tmp);
+    }
+  } else {
+    const Idx n = nf * a.st_per;
+    for (Idx t = tid; t < n; t += nt) {
+      const Idx k = a.fst.div(t);
+      const Idx rem = t - k * a.st_per;
+      const Idx q = a.wu == 1 ? rem : a.fwu.div(rem);
+      const Idx x = rem - q * a.wu;
+      const Idx i = a.fW.div(q);
+      const Idx jj = q - i * a.W;
+      const Foot<Idx, OP_COL> f = decode<Idx, OP_COL>(a, f0 + k);
+      if (jj >= f.weff) continue;
+      const Idx s = col_src<Idx, MAP>(a, i, f.row + jj);
+      *reinterpret_cast<Word*>(a.data + f.addr + (uint64_t)i * a.CU + (uint64_t)jj * a.U + x * w) =
+          *reinterpret_cast<const Word*>(lds + k * a.fpl + s * a.rs + jj * (Idx)a.us + x * w);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// row pass: K rows of C cells per workgroup; LDS row k at k*rs
+// ---------------------------------------------------------------------------
+template <typename Word, typename Idx, int MAP, int VEC>
+__global__ __launch_bounds__(512) void k_row(Args<Idx> a, int remap) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const Idx wg = (Idx)xcd_logical(blockIdx.x, gridDim.x, remap);
+  const Idx f0 = wg * a.K;
+  const Idx nf = (a.nfp - f0) < a.K ? (a.nfp - f0) : a.K;
+  const Idx tid = threadIdx.x, nt = blockDim.x;
+  constexpr Idx w = sizeof(Word);
+
+  if (a.cpr) {
+    // S == 1: the nf rows are one contiguous byte range (rs = C*U, no padding)
+    const W16* g = reinterpret_cast<const W16*>(a.data + a.base + (uint64_t)f0 * a.CU);
+    const Idx n = nf * a.cpr;
+    for (Idx t0 = tid; t0 < n; t0 += nt * UNR) {
+      W16 v[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const Idx t = t0 + (Idx)u * nt;
+        if (t < n) v[u] = g[t];
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const Idx t = t0 + (Idx)u * nt;
+        if (t < n) reinterpret_cast<W16*>(lds)[t] = v[u];
+      }
+    }
+  } else {
+    const Idx n = nf * a.ld_per;
+    for (Idx t0 = tid; t0 < n; t0 += nt * UNR) {
+      Word v[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const Idx t = t0 + (Idx)u * nt;
+        if (t < n) {
+          const Idx k = a.fld.div(t);
+          const Idx rem = t - k * a.ld_per;
+          const Idx q = a.wu == 1 ? rem : a.fwu.div(rem);
+          const Idx x = rem - q * a.wu;
+          const Foot<Idx, OP_ROW> f = decode<Idx, OP_ROW>(a, f0 + k);
+          v[u] = *reinterpret_cast<const Word*>(a.data + f.addr + (uint64_t)q * a.U + x * w);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const Idx t = t0 + (Idx)u * nt;
+        if (t < n) reinterpret_cast<Word*>(lds)[t] = v[u];
+      }
+    }
+  }
+  __syncthreads();
+
+  if (VEC > 1) {
+    // S == 1, one word per cell: VEC consecutive destination cells per store
+    const Idx n = nf * a.st_per;
+    char* gbase = a.data + a.base + (uint64_t)f0 * a.CU;
+    for (Idx t = tid; t < n; t += nt) {
+      const Idx k = a.fst.div(t);
+      const Idx j0 = (t - k * a.st_per) * VEC;
+      const Idx i = a.fR.mod(f0 + k);
+      Word tmp[VEC];
+#pragma unroll
+      for (int v = 0; v < VEC; ++v)
+        tmp[v] = reinterpret_cast<const Word*>(lds)[k * a.C + row_src<Idx, MAP>(a, i, j0 + v)];
+      *reinterpret_cast<Vec<VEC * sizeof(Word)>*>(gbase + (uint64_t)k * a.CU + (uint64_t)j0 * w) =
+          *reinterpret_cast<const Vec<VEC * sizeof(Word)>*>(tmp);
+    }
+  } else {
+    const Idx n = nf * a.st_per;
+    for (Idx t = tid; t < n; t += nt) {
+      const Idx k = a.fst.div(t);
+      const Idx rem = t - k * a.st_per;
+      const Idx q = a.wu == 1 ? rem : a.fwu.div(rem);
+      const Idx x = rem - q * a.wu;
+      const Foot<Idx, OP_ROW> f = decode<Idx, OP_ROW>(a, f0 + k);
+      const Idx s = row_src<Idx, MAP>(a, f.row, q);
+      *reinterpret_cast<Word*>(a.data + f.addr + (uint64_t)q * a.U + x * w) =
+          reinterpret_cast<const Word*>(lds)[k * a.st_per + s * a.wu + x];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// staged passes
+// ---------------------------------------------------------------------------
+
+// copy footprints [fp0, fp0+G) to scratch, footprint-major (cells in
+// footprint order); for OP_TILE with S == 1 this is a contiguous copy
 template <typename Word, typename Idx, int KIND>
-__global__ __launch_bounds__(256) void k_stage_out(PassArgs<Idx> a) {
-  const uint64_t total = a.G * (uint64_t)a.per_fp;
+__global__ __launch_bounds__(256) void k_stage_out(Args<Idx> a) {
+  constexpr Idx w = sizeof(Word);
+  const Idx NU = KIND == OP_TILE ? a.R * a.C : (KIND == OP_COL ? a.R * a.W : a.C);
+  const uint64_t per = (uint64_t)NU * a.wu;
+  const uint64_t total = a.G * per;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   Word* sc = reinterpret_cast<Word*>(a.scratch);
-  for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
-    const uint64_t fl = idx / a.per_fp;
-    const Idx rem = (Idx)(idx - fl * a.per_fp);
-    const Idx q = a.wu == 1 ? rem : a.fwu.div(rem);
-    const Idx x = rem - q * a.wu;
-    const Foot<Idx, KIND> f = decode<Idx, KIND>(a, (Idx)(a.fp0 + fl));
-    Idx col;
-    const uint64_t off = unit_off<Idx, KIND>(a, q, &col);
-    if (KIND == OP_COL && col >= f.weff) continue;
-    sc[idx] = *reinterpret_cast<const Word*>(a.data + f.addr + off + (uint64_t)x * sizeof(Word));
+  for (uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t0 < total;
+       t0 += stride * UNR) {
+    Word v[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const uint64_t t = t0 + u * stride;
+      if (t >= total) continue;
+      const uint64_t fl = t / per;
+      const Idx rem = (Idx)(t - fl * per);
+      const Idx q = a.wu == 1 ? rem : a.fwu.div(rem);
+      const Idx x = rem - q * a.wu;
+      const Foot<Idx, KIND> f = decode<Idx, KIND>(a, (Idx)(a.fp0 + fl));
+      uint64_t off;
+      if (KIND == OP_COL) {
+        const Idx i = a.fW.div(q);
+        const Idx jj = q - i * a.W;
+        if (jj >= f.weff) { v[u] = Word(); continue; }
+        off = (uint64_t)i * a.CU + (uint64_t)jj * a.U;
+      } else {
+        off = (uint64_t)q * a.U;
+      }
+      v[u] = *reinterpret_cast<const Word*>(a.data + f.addr + off + x * w);
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const uint64_t t = t0 + u * stride;
+      if (t < total) sc[t] = v[u];
+    }
   }
 }
 
-// staged pass, part 2: gather from scratch back into the footprints
+// gather the footprints back from scratch through the column/row maps
 template <typename Word, typename Idx, int KIND, int MAP>
-__global__ __launch_bounds__(256) void k_stage_in(PassArgs<Idx> a) {
-  const uint64_t total = a.G * (uint64_t)a.per_fp;
+__global__ __launch_bounds__(256) void k_stage_in(Args<Idx> a) {
+  constexpr Idx w = sizeof(Word);
+  const Idx NU = KIND == OP_COL ? a.R * a.W : a.C;
+  const uint64_t per = (uint64_t)NU * a.wu;
+  const uint64_t total = a.G * per;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const Word* sc = reinterpret_cast<const Word*>(a.scratch);
-  for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
-    const uint64_t fl = idx / a.per_fp;
-    const Idx rem = (Idx)(idx - fl * a.per_fp);
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
+    const uint64_t fl = t / per;
+    const Idx rem = (Idx)(t - fl * per);
     const Idx q = a.wu == 1 ? rem : a.fwu.div(rem);
     const Idx x = rem - q * a.wu;
     const Foot<Idx, KIND> f = decode<Idx, KIND>(a, (Idx)(a.fp0 + fl));
-    Idx col;
-    const uint64_t off = unit_off<Idx, KIND>(a, q, &col);
-    if (KIND == OP_COL && col >= f.weff) continue;
-    const Idx s = src_unit<Idx, KIND, MAP>(a, f, q);
-    *reinterpret_cast<Word*>(a.data + f.addr + off + (uint64_t)x * sizeof(Word)) =
-        sc[fl * a.per_fp + (uint64_t)s * a.wu + x];
+    uint64_t off;
+    Idx s;
+    if (KIND == OP_COL) {
+      const Idx i = a.fW.div(q);
+      const Idx jj = q - i * a.W;
+      if (jj >= f.weff) continue;
+      off = (uint64_t)i * a.CU + (uint64_t)jj * a.U;
+      s = col_src<Idx, MAP>(a, i, f.row + jj) * a.W + jj;
+    } else {
+      off = (uint64_t)q * a.U;
+      s = row_src<Idx, MAP>(a, f.row, q);
+    }
+    *reinterpret_cast<Word*>(a.data + f.addr + off + x * w) = sc[fl * per + (uint64_t)s * a.wu + x];
+  }
+}
+
+// staged tile, part 2: out-of-place LDS-tiled transpose of the footprints
+// held in scratch (each an R x C row-major matrix of us-byte cells) back into
+// the tensor, where output cell (j, i) of footprint f lives at
+// addr(f) + (j*R + i)*U.  One workgroup per (footprint, TR x TC tile); TR and
+// TC are powers of two.  Loads move 16-byte chunks of tile rows; stores pack
+// VEC small cells (or move 16-byte chunks of large cells) along output rows.
+struct TileGeom {
+  uint32_t tr_log2, tc_log2;  // tile edge logs
+  uint32_t tiles_r, tiles_c;  // tiles per footprint
+  uint32_t rs;                // padded LDS row stride (bytes)
+  uint32_t cpr;               // 16-byte chunks per full tile row (0: word loads)
+};
+
+template <typename Word, typename Idx, int VEC>
+__global__ __launch_bounds__(256) void k_tile_oop(Args<Idx> a, TileGeom tg) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  constexpr Idx w = sizeof(Word);
+  const Idx tid = threadIdx.x, nt = blockDim.x;
+  const uint64_t per_fp = (uint64_t)tg.tiles_r * tg.tiles_c;
+  const uint64_t tile = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x;
+  if (tile >= a.G * per_fp) return;
+  const uint64_t fl = tile / per_fp;
+  const uint32_t tt = (uint32_t)(tile - fl * per_fp);
+  const uint32_t ti = tt / tg.tiles_c, tj = tt - ti * tg.tiles_c;
+  const Idx TR = (Idx)1 << tg.tr_log2, TC = (Idx)1 << tg.tc_log2;
+  const Idx i0 = ti * TR, j0 = tj * TC;
+  const Idx nr = (a.R - i0) < TR ? (a.R - i0) : TR;
+  const Idx nc = (a.C - j0) < TC ? (a.C - j0) : TC;
+  const unsigned char* src =
+      reinterpret_cast<const unsigned char*>(a.src) + fl * (uint64_t)a.R * a.C * a.us;
+  // ---- load tile rows i0..i0+nr, cells j0..j0+nc (nc*us contiguous bytes each)
+  if (tg.cpr && nc == TC) {
+    const Idx n = nr * tg.cpr;
+    for (Idx t0 = tid; t0 < n; t0 += nt * UNR) {
+      W16 v[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const Idx t = t0 + (Idx)u * nt;
+        if (t < n) {
+          const Idx r = t / tg.cpr, c = t - (t / tg.cpr) * tg.cpr;
+          v[u] = *reinterpret_cast<const W16*>(src + ((uint64_t)(i0 + r) * a.C + j0) * a.us + c * 16);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const Idx t = t0 + (Idx)u * nt;
+        if (t < n) {
+          const Idx r = t / tg.cpr, c = t - (t / tg.cpr) * tg.cpr;
+          lds_put16(lds + r * tg.rs + c * 16, v[u]);
+        }
+      }
+    }
+  } else {
+    const Idx wpr = nc * a.wu;  // words per tile row
+    const Idx n = nr * wpr;
+    for (Idx t = tid; t < n; t += nt) {
+      const Idx r = t / wpr, c = t - (t / wpr) * wpr;
+      *reinterpret_cast<Word*>(lds + r * tg.rs + c * w) =
+          *reinterpret_cast<const Word*>(src + ((uint64_t)(i0 + r) * a.C + j0) * a.us + c * w);
+    }
+  }
+  __syncthreads();
+  // ---- store output rows j0..j0+nc, cells i0..i0+nr
+  const Foot<Idx, OP_TILE> f = decode<Idx, OP_TILE>(a, (Idx)(a.fp0 + fl));
+  if (VEC > 1 && nr == TR) {
+    // cells of one word, contiguous output rows (S == 1): VEC cells per store
+    const Idx vpr = TR / VEC;
+    const Idx n = nc * vpr;
+    for (Idx t = tid; t < n; t += nt) {
+      const Idx jr = t / vpr, iv = (t - (t / vpr) * vpr) * VEC;
+      Word tmp[VEC];
+#pragma unroll
+      for (int v = 0; v < VEC; ++v)
+        tmp[v] = *reinterpret_cast<const Word*>(lds + (iv + v) * tg.rs + jr * w);
+      *reinterpret_cast<Vec<VEC * sizeof(Word)>*>(
+          a.data + f.addr + ((uint64_t)(j0 + jr) * a.R + i0 + iv) * a.U) =
+          *reinterpret_cast<const Vec<VEC * sizeof(Word)>*>(tmp);
+    }
+  } else {
+    const Idx cw = a.wu;
+    const Idx n = nc * nr * cw;
+    const Idx per_row = nr * cw;
+    for (Idx t = tid; t < n; t += nt) {
+      const Idx jr = t / per_row, rem = t - (t / per_row) * per_row;
+      const Idx ir = rem / cw, x = rem - (rem / cw) * cw;
+      *reinterpret_cast<Word*>(a.data + f.addr + ((uint64_t)(j0 + jr) * a.R + i0 + ir) * a.U +
+                               x * w) =
+          *reinterpret_cast<const Word*>(lds + ir * tg.rs + jr * a.us + x * w);
+    }
   }
 }
 
@@ -212,10 +632,9 @@ __global__ __launch_bounds__(256) void k_stage_in(PassArgs<Idx> a) {
 // ---------------------------------------------------------------------------
 
 template <typename Idx>
-PassArgs<Idx> make_args(const Op& op, char* data, int64_t wbytes) {
-  PassArgs<Idx> a{};
+Args<Idx> make_args(const Op& op, char* data, int64_t wbytes) {
+  Args<Idx> a{};
   a.data = data;
-  a.scratch = nullptr;
   a.base = op.base;
   a.U = op.U;
   a.us = op.us;
@@ -225,21 +644,16 @@ PassArgs<Idx> make_args(const Op& op, char* data, int64_t wbytes) {
   a.C = (Idx)op.C;
   a.W = (Idx)(op.kind == OP_COL ? op.W : 1);
   a.S = (Idx)(op.U / op.us);
-  a.NU = (Idx)op.footprint_units();
-  a.wu = (Idx)(op.us / wbytes);
-  a.per_fp = a.NU * a.wu;
   a.nfp = (Idx)op.num_footprints();
   a.P = (Idx)((op.C + a.W - 1) / a.W);
+  a.wu = (Idx)(op.us / wbytes);
   a.K = 1;
-  a.fp0 = 0;
-  a.G = 0;
-  a.contiguous = 0;
   a.fS.init(a.S);
   a.fR.init(op.R);
+  a.fC.init(op.C);
   a.fW.init(a.W);
   a.fP.init(a.P);
   a.fwu.init(a.wu);
-  a.fper.init(a.per_fp);
   if (op.kind != OP_TILE) a.g.init(op.m, op.n, op.c, op.a, op.b, op.k, op.binv);
   return a;
 }
@@ -247,9 +661,9 @@ PassArgs<Idx> make_args(const Op& op, char* data, int64_t wbytes) {
 bool needs64(const Op& op) {
   const unsigned __int128 lim = (unsigned __int128)1 << 31;
   auto big = [&](unsigned __int128 v) { return v >= lim; };
-  if (big((unsigned __int128)op.num_footprints())) return true;
-  if (big((unsigned __int128)op.footprint_units() * (op.us))) return true;
-  if (op.kind == OP_TILE) return big((unsigned __int128)op.R * op.C);
+  if (big((unsigned __int128)op.num_footprints() * 2)) return true;
+  if (big((unsigned __int128)op.footprint_units() * op.us * 2)) return true;
+  if (op.kind == OP_TILE) return big((unsigned __int128)op.R * op.C * 2);
   const unsigned __int128 m = op.m, n = op.n, a = op.a, b = op.b, c = op.c;
   return big(m * m + n) || big(b * b) || big(a * a) || big(a * b) || big(n + 2 * m) ||
          big(c * b + m) || big(m * 3);
@@ -261,58 +675,208 @@ int64_t word_bytes(const Op& op, uintptr_t data) {
   return w;
 }
 
-template <typename Word, typename Idx, int KIND, int MAP>
-hipError_t run_typed(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
-                     const LaunchCfg& cfg, hipStream_t st) {
-  PassArgs<Idx> a = make_args<Idx>(op, data, sizeof(Word));
-  const int64_t fpb = (int64_t)a.per_fp * (int64_t)sizeof(Word);
-  if (!op.staged) {
-    int64_t K = std::max<int64_t>(1, cfg.lds_target / std::max<int64_t>(fpb, 1));
-    // keep at least ~4 workgroups per CU in flight when there is enough work
-    const int64_t nfp = (int64_t)a.nfp;
-    const int64_t want = 1024;
-    if (nfp / K < want) K = std::max<int64_t>(1, nfp / want);
-    a.K = (Idx)K;
-    a.fper.init(a.per_fp);
-    a.contiguous = (KIND != OP_COL && a.S == 1) ? 1 : 0;
-    const int64_t nwg = (nfp + K - 1) / K;
-    const size_t lds = (size_t)(K * fpb);
-    hipLaunchKernelGGL((k_lds<Word, Idx, KIND, MAP>), dim3((uint32_t)nwg), dim3(cfg.threads), lds,
-                       st, a, cfg.xcd_remap);
-    return hipGetLastError();
+// workgroups want at least this many footprints in flight across the GPU
+constexpr int64_t kMinGroups = 1024;
+
+int64_t pick_K(int64_t nfp, int64_t fpb, int64_t lds_target) {
+  int64_t K = std::max<int64_t>(1, lds_target / std::max<int64_t>(fpb, 1));
+  if (nfp / K < kMinGroups) K = std::max<int64_t>(1, nfp / kMinGroups);
+  return K;
+}
+
+template <typename Word, typename Idx, int VEC>
+hipError_t launch_tile(Args<Idx> a, const Op& op, const LaunchCfg& cfg, hipStream_t st) {
+  const int64_t w = sizeof(Word);
+  const int64_t rowbytes = op.C * op.us;
+  a.rs = (Idx)tile_row_stride(rowbytes, w);
+  a.fpl = (Idx)(op.R * a.rs);
+  const bool chunk16 = a.S == 1 && rowbytes % 16 == 0 &&
+                       ((uintptr_t)(a.data + a.base) % 16) == 0 && (a.rs % 4) == 0;
+  a.cpr = chunk16 ? (Idx)(rowbytes / 16) : 0;
+  a.fcpr.init(a.cpr ? a.cpr : 1);
+  const bool cells16 = !chunk16 && VEC == 1 && w == 16 && op.us % 16 == 0 && op.U % 16 == 0 &&
+                       ((uintptr_t)(a.data + a.base) % 16) == 0 && a.rs % 16 == 0;
+  a.cps = cells16 ? (Idx)(op.us / 16) : 0;
+  a.cps_log2 = -1;
+  if (a.cps && (a.cps & (a.cps - 1)) == 0) {
+    a.cps_log2 = 0;
+    while (((Idx)1 << a.cps_log2) < a.cps) ++a.cps_log2;
   }
+  a.fcps.init(a.cps ? a.cps : 1);
+  a.ld_per = a.cps ? a.R * a.C * a.cps : a.R * a.C * a.wu;
+  a.fld.init(a.ld_per);
+  if (VEC > 1) {
+    a.vpr = (Idx)(op.R / VEC);
+    a.st_per = (Idx)(op.C * a.vpr);
+  } else {
+    a.vpr = 1;
+    a.st_per = a.R * a.C * a.wu;
+  }
+  a.fst.init(a.st_per);
+  a.fvpr.init(a.vpr);
+  const int64_t K = pick_K((int64_t)a.nfp, (int64_t)a.fpl, cfg.lds_target);
+  a.K = (Idx)K;
+  const int64_t nwg = ((int64_t)a.nfp + K - 1) / K;
+  const int threads = K * a.fpl > 96 * 1024 ? cfg.threads_big : cfg.threads;
+  hipLaunchKernelGGL((k_tile<Word, Idx, VEC>), dim3((uint32_t)nwg), dim3(threads),
+                     (size_t)(K * a.fpl), st, a, cfg.xcd_remap);
+  return hipGetLastError();
+}
+
+template <typename Word, typename Idx, int MAP, int VEC>
+hipError_t launch_col(Args<Idx> a, const Op& op, const LaunchCfg& cfg, hipStream_t st) {
+  const int64_t seg = op.W * op.us;
+  a.rs = (Idx)seg;
+  a.fpl = (Idx)(op.R * seg);
+  const bool full = op.C % op.W == 0;
+  const bool chunk16 = full && seg % 16 == 0 && ((uintptr_t)(a.data + a.base) % 16) == 0 &&
+                       (a.CU % 16) == 0 && (op.us == op.U || op.us % 16 == 0) &&
+                       ((op.us == op.U) ? true : op.W == 1);
+  a.cpr = chunk16 ? (Idx)(seg / 16) : 0;
+  a.fcpr.init(a.cpr ? a.cpr : 1);
+  a.ld_per = chunk16 ? a.R * a.cpr : a.R * a.W * a.wu;
+  a.fld.init(a.ld_per);
+  if (VEC > 1) {
+    a.vpr = (Idx)(op.W / VEC);
+    a.st_per = a.R * a.vpr;
+  } else {
+    a.vpr = 1;
+    a.st_per = a.R * a.W * a.wu;
+  }
+  a.fst.init(a.st_per);
+  a.fvpr.init(a.vpr);
+  const int64_t K = pick_K((int64_t)a.nfp, (int64_t)a.fpl, cfg.lds_target);
+  a.K = (Idx)K;
+  const int64_t nwg = ((int64_t)a.nfp + K - 1) / K;
+  const int threads = K * a.fpl > 96 * 1024 ? cfg.threads_big : cfg.threads;
+  hipLaunchKernelGGL((k_col<Word, Idx, MAP, VEC>), dim3((uint32_t)nwg), dim3(threads),
+                     (size_t)(K * a.fpl), st, a, cfg.xcd_remap);
+  return hipGetLastError();
+}
+
+template <typename Word, typename Idx, int MAP, int VEC>
+hipError_t launch_row(Args<Idx> a, const Op& op, const LaunchCfg& cfg, hipStream_t st) {
+  a.rs = (Idx)(op.C * op.us);
+  a.fpl = a.rs;
+  const bool chunk16 = a.S == 1 && (a.rs % 16) == 0 && ((uintptr_t)(a.data + a.base) % 16) == 0;
+  a.cpr = chunk16 ? (Idx)(a.rs / 16) : 0;
+  a.ld_per = a.C * a.wu;
+  a.fld.init(a.ld_per);
+  if (VEC > 1) {
+    a.vpr = (Idx)(op.C / VEC);
+    a.st_per = a.vpr;
+  } else {
+    a.vpr = 1;
+    a.st_per = a.C * a.wu;
+  }
+  a.fst.init(a.st_per);
+  a.fvpr.init(a.vpr);
+  const int64_t K = pick_K((int64_t)a.nfp, (int64_t)a.fpl, cfg.lds_target);
+  a.K = (Idx)K;
+  const int64_t nwg = ((int64_t)a.nfp + K - 1) / K;
+  const int threads = K * a.fpl > 96 * 1024 ? cfg.threads_big : cfg.threads;
+  hipLaunchKernelGGL((k_row<Word, Idx, MAP, VEC>), dim3((uint32_t)nwg), dim3(threads),
+                     (size_t)(K * a.fpl), st, a, cfg.xcd_remap);
+  return hipGetLastError();
+}
+
+template <typename Word, typename Idx, int KIND, int MAP>
+hipError_t launch_staged(Args<Idx> a, const Op& op, char* scratch, int64_t scratch_bytes,
+                         hipStream_t st) {
+  const int64_t fpb = op.footprint_units() * op.us;
   if (!scratch || scratch_bytes < fpb) return hipErrorInvalidValue;
   a.scratch = scratch;
+  a.src = scratch;
   const uint64_t nfp = (uint64_t)a.nfp;
   const uint64_t G = std::min<uint64_t>(nfp, (uint64_t)(scratch_bytes / fpb));
   for (uint64_t f0 = 0; f0 < nfp; f0 += G) {
     a.fp0 = f0;
     a.G = std::min<uint64_t>(G, nfp - f0);
-    const uint64_t words = a.G * (uint64_t)a.per_fp;
+    const uint64_t words = a.G * (uint64_t)op.footprint_units() * a.wu;
     const uint64_t blocks = std::min<uint64_t>((words + 255) / 256, 256ull * 64);
-    hipLaunchKernelGGL((k_stage_out<Word, Idx, KIND>), dim3((uint32_t)blocks), dim3(256), 0, st, a);
-    hipLaunchKernelGGL((k_stage_in<Word, Idx, KIND, MAP>), dim3((uint32_t)blocks), dim3(256), 0, st,
-                       a);
+    if (KIND == OP_TILE && a.S == 1) {
+      // whole matrices: one contiguous range
+      hipError_t e = hipMemcpyAsync(scratch, a.data + a.base + f0 * a.RCU, a.G * a.RCU,
+                                    hipMemcpyDeviceToDevice, st);
+      if (e != hipSuccess) return e;
+    } else {
+      hipLaunchKernelGGL((k_stage_out<Word, Idx, KIND>), dim3((uint32_t)blocks), dim3(256), 0, st, a);
+    }
+    if (KIND == OP_TILE) {
+      // tile edges: about 16 KiB of cells, 16-byte rows at least
+      uint32_t trl = 6, tcl = 6;
+      while (trl > 0 && ((int64_t)1 << (trl + tcl)) * op.us > 16384) {
+        --trl;
+        --tcl;
+      }
+      while (trl > 0 && ((int64_t)1 << (trl - 1)) >= op.R) --trl;
+      while (tcl > 0 && ((int64_t)1 << (tcl - 1)) >= op.C) --tcl;
+      TileGeom tg;
+      tg.tr_log2 = trl;
+      tg.tc_log2 = tcl;
+      tg.tiles_r = (uint32_t)((op.R + (1 << trl) - 1) >> trl);
+      tg.tiles_c = (uint32_t)((op.C + (1 << tcl) - 1) >> tcl);
+      const int64_t rowb = ((int64_t)1 << tcl) * op.us;
+      tg.rs = (uint32_t)tile_row_stride(rowb, sizeof(Word));
+      tg.cpr = (rowb % 16 == 0 && (op.C * op.us) % 16 == 0 && tg.rs % 4 == 0) ? (uint32_t)(rowb / 16) : 0;
+      const uint64_t ntiles = a.G * (uint64_t)tg.tiles_r * tg.tiles_c;
+      const size_t lds = (size_t)((int64_t)1 << trl) * tg.rs;
+      const uint32_t gx = (uint32_t)std::min<uint64_t>(ntiles, 65535);
+      const uint32_t gy = (uint32_t)((ntiles + gx - 1) / gx);
+      constexpr int V = 16 / sizeof(Word) > 1 ? 16 / sizeof(Word) : 1;
+      const bool vec = V > 1 && a.wu == 1 && a.S == 1 && ((int64_t)1 << trl) % V == 0 &&
+                       op.R % V == 0 && ((uintptr_t)(a.data + a.base) % 16) == 0;
+      if (vec)
+        hipLaunchKernelGGL((k_tile_oop<Word, Idx, V>), dim3(gx, gy), dim3(256), lds, st, a, tg);
+      else
+        hipLaunchKernelGGL((k_tile_oop<Word, Idx, 1>), dim3(gx, gy), dim3(256), lds, st, a, tg);
+    } else {
+      hipLaunchKernelGGL((k_stage_in<Word, Idx, KIND, MAP>), dim3((uint32_t)blocks), dim3(256), 0, st,
+                         a);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
 }
 
+template <typename Word, typename Idx, int KIND, int MAP>
+hipError_t run_kind(const Op& op, char* d, char* s, int64_t sb, const LaunchCfg& c, hipStream_t st) {
+  Args<Idx> a = make_args<Idx>(op, d, sizeof(Word));
+  if (op.staged) return launch_staged<Word, Idx, KIND, MAP>(a, op, s, sb, st);
+  constexpr int V = 16 / sizeof(Word) > 1 ? 16 / sizeof(Word) : 1;
+  const bool one_word = a.wu == 1 && a.S == 1;
+  const uintptr_t base = (uintptr_t)(d + op.base);
+  if (KIND == OP_TILE) {
+    if (V > 1 && one_word && op.R % V == 0 && base % 16 == 0)
+      return launch_tile<Word, Idx, V>(a, op, c, st);
+    return launch_tile<Word, Idx, 1>(a, op, c, st);
+  }
+  if (KIND == OP_COL) {
+    if (V > 1 && one_word && op.W % V == 0 && op.C % op.W == 0 && base % 16 == 0 &&
+        (op.C * op.U) % 16 == 0)
+      return launch_col<Word, Idx, MAP, V>(a, op, c, st);
+    return launch_col<Word, Idx, MAP, 1>(a, op, c, st);
+  }
+  if (V > 1 && one_word && op.C % V == 0 && base % 16 == 0)
+    return launch_row<Word, Idx, MAP, V>(a, op, c, st);
+  return launch_row<Word, Idx, MAP, 1>(a, op, c, st);
+}
+
 template <typename Word, typename Idx>
 hipError_t run_map(const Op& op, char* d, char* s, int64_t sb, const LaunchCfg& c, hipStream_t st) {
-  if (op.kind == OP_TILE) return run_typed<Word, Idx, OP_TILE, MAP_NONE>(op, d, s, sb, c, st);
+  if (op.kind == OP_TILE) return run_kind<Word, Idx, OP_TILE, MAP_NONE>(op, d, s, sb, c, st);
   if (op.kind == OP_COL) {
     switch (op.map) {
-      case MAP_ROT: return run_typed<Word, Idx, OP_COL, MAP_ROT>(op, d, s, sb, c, st);
-      case MAP_ROTINV: return run_typed<Word, Idx, OP_COL, MAP_ROTINV>(op, d, s, sb, c, st);
-      case MAP_COLSH: return run_typed<Word, Idx, OP_COL, MAP_COLSH>(op, d, s, sb, c, st);
-      case MAP_COLSHINV: return run_typed<Word, Idx, OP_COL, MAP_COLSHINV>(op, d, s, sb, c, st);
+      case MAP_ROT: return run_kind<Word, Idx, OP_COL, MAP_ROT>(op, d, s, sb, c, st);
+      case MAP_ROTINV: return run_kind<Word, Idx, OP_COL, MAP_ROTINV>(op, d, s, sb, c, st);
+      case MAP_COLSH: return run_kind<Word, Idx, OP_COL, MAP_COLSH>(op, d, s, sb, c, st);
+      case MAP_COLSHINV: return run_kind<Word, Idx, OP_COL, MAP_COLSHINV>(op, d, s, sb, c, st);
     }
   }
   if (op.kind == OP_ROW) {
-    if (op.map == MAP_ROWG) return run_typed<Word, Idx, OP_ROW, MAP_ROWG>(op, d, s, sb, c, st);
-    if (op.map == MAP_ROWD) return run_typed<Word, Idx, OP_ROW, MAP_ROWD>(op, d, s, sb, c, st);
+    if (op.map == MAP_ROWG) return run_kind<Word, Idx, OP_ROW, MAP_ROWG>(op, d, s, sb, c, st);
+    if (op.map == MAP_ROWD) return run_kind<Word, Idx, OP_ROW, MAP_ROWD>(op, d, s, sb, c, st);
   }
   return hipErrorInvalidValue;
 }
@@ -326,6 +890,13 @@ hipError_t run_idx(const Op& op, char* d, char* s, int64_t sb, const LaunchCfg& 
 // ---- pattern fill / verify ----------------------------------------------------
 
 __device__ __forceinline__ uint64_t sm64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+uint64_t sm64_host(uint64_t x) {
   x += 0x9E3779B97F4A7C15ull;
   x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
   x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
@@ -377,11 +948,10 @@ __global__ void k_verify(const T* p, VerifyArgs v, uint64_t seed_mixed, int mode
 
 int64_t op_lds_bytes(const Op& op, const LaunchCfg& cfg) {
   if (op.staged || op.kind == 0) return 0;
-  const int64_t fpb = op.footprint_units() * op.us;
-  int64_t K = std::max<int64_t>(1, cfg.lds_target / std::max<int64_t>(fpb, 1));
-  const int64_t nfp = op.num_footprints();
-  if (nfp / K < 1024) K = std::max<int64_t>(1, nfp / 1024);
-  return K * fpb;
+  int64_t fpb;
+  if (op.kind == OP_TILE) fpb = op.R * tile_row_stride(op.C * op.us, 16);
+  else fpb = op.footprint_units() * op.us;
+  return pick_K(op.num_footprints(), fpb, cfg.lds_target) * fpb;
 }
 
 hipError_t launch_op(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
@@ -398,12 +968,7 @@ hipError_t launch_op(const Op& op, char* data, char* scratch, int64_t scratch_by
 
 hipError_t launch_fill(char* data, uint64_t n, int elem, uint64_t seed, int mode, uint64_t first,
                        hipStream_t stream) {
-  const uint64_t mixed = [&] {
-    uint64_t x = seed + 0x9E3779B97F4A7C15ull;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    return x ^ (x >> 31);
-  }();
+  const uint64_t mixed = sm64_host(seed);
   const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 256ull * 128);
   if (n == 0) return hipSuccess;
   switch (elem) {
@@ -433,12 +998,7 @@ hipError_t launch_verify(const char* data, const int64_t* dims, const int* perm0
     vol *= (uint64_t)dims[k];
   }
   v.vol = vol;
-  const uint64_t mixed = [&] {
-    uint64_t x = seed + 0x9E3779B97F4A7C15ull;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    return x ^ (x >> 31);
-  }();
+  const uint64_t mixed = sm64_host(seed);
   const uint32_t blocks = (uint32_t)std::min<uint64_t>((vol + 255) / 256, 256ull * 128);
   switch (elem) {
     case 1: hipLaunchKernelGGL(k_verify<uint8_t>, dim3(blocks), dim3(256), 0, stream, (const uint8_t*)data, v, mixed, mode, d_mismatch); break;
@@ -451,36 +1011,46 @@ hipError_t launch_verify(const char* data, const int64_t* dims, const int* perm0
 }
 
 namespace {
-template <typename Word, typename Idx>
-hipError_t cfg_words(int64_t bytes) {
-  hipError_t e = hipSuccess;
+template <typename Word, typename Idx, int V>
+void cfg_set(int64_t bytes, hipError_t& e) {
   auto set = [&](const void* fn) {
     hipError_t r = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (r != hipSuccess) e = r;
   };
-  set((const void*)k_lds<Word, Idx, OP_TILE, MAP_NONE>);
-  set((const void*)k_lds<Word, Idx, OP_COL, MAP_ROT>);
-  set((const void*)k_lds<Word, Idx, OP_COL, MAP_ROTINV>);
-  set((const void*)k_lds<Word, Idx, OP_COL, MAP_COLSH>);
-  set((const void*)k_lds<Word, Idx, OP_COL, MAP_COLSHINV>);
-  set((const void*)k_lds<Word, Idx, OP_ROW, MAP_ROWG>);
-  set((const void*)k_lds<Word, Idx, OP_ROW, MAP_ROWD>);
-  return e;
+  set((const void*)k_tile<Word, Idx, V>);
+  set((const void*)k_col<Word, Idx, MAP_ROT, V>);
+  set((const void*)k_col<Word, Idx, MAP_ROTINV, V>);
+  set((const void*)k_col<Word, Idx, MAP_COLSH, V>);
+  set((const void*)k_col<Word, Idx, MAP_COLSHINV, V>);
+  set((const void*)k_row<Word, Idx, MAP_ROWG, V>);
+  set((const void*)k_row<Word, Idx, MAP_ROWD, V>);
 }
 template <typename Word>
-hipError_t cfg_idx(int64_t bytes) {
-  hipError_t e = cfg_words<Word, uint32_t>(bytes);
-  hipError_t f = cfg_words<Word, uint64_t>(bytes);
-  return e != hipSuccess ? e : f;
+void cfg_word(int64_t bytes, hipError_t& e) {
+  constexpr int V = 16 / sizeof(Word) > 1 ? 16 / sizeof(Word) : 1;
+  cfg_set<Word, uint32_t, 1>(bytes, e);
+  cfg_set<Word, uint64_t, 1>(bytes, e);
+  if (V > 1) {
+    cfg_set<Word, uint32_t, V>(bytes, e);
+    cfg_set<Word, uint64_t, V>(bytes, e);
+  }
+  auto set = [&](const void* fn) {
+    hipError_t r = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (r != hipSuccess) e = r;
+  };
+  set((const void*)k_tile_oop<Word, uint32_t, 1>);
+  set((const void*)k_tile_oop<Word, uint64_t, 1>);
 }
 }  // namespace
 
 hipError_t configure_kernels(int64_t bytes) {
-  hipError_t e[5] = {cfg_idx<W16>(bytes), cfg_idx<uint64_t>(bytes), cfg_idx<uint32_t>(bytes),
-                     cfg_idx<uint16_t>(bytes), cfg_idx<uint8_t>(bytes)};
-  for (hipError_t x : e)
-    if (x != hipSuccess) return x;
-  return hipSuccess;
+  hipError_t e = hipSuccess;
+  cfg_word<W16>(bytes, e);
+  cfg_word<uint64_t>(bytes, e);
+  cfg_word<uint32_t>(bytes, e);
+  cfg_word<uint16_t>(bytes, e);
+  cfg_word<uint8_t>(bytes, e);
+  return e;
 }
 
 }  // namespace eithot

@@ -12,6 +12,7 @@ namespace eithot {
 struct LaunchCfg {
   int64_t lds_target = 64 * 1024;  // footprints are packed into one workgroup up to this
   int threads = 256;
+  int threads_big = 512;           // workgroups whose LDS footprint exceeds 96 KiB
   int xcd_remap = 1;               // keep consecutive footprints on one XCD
 };
 

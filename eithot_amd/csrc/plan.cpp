@@ -233,6 +233,20 @@ int64_t pick_slice(int64_t units, int64_t U, int64_t budget, int64_t min_seg) {
   return best;
 }
 
+// Tile footprint (R rows of C slices, padded rows) that fits `budget`.
+int64_t pick_tile_slice(int64_t R, int64_t C, int64_t U, int64_t budget, int64_t min_seg) {
+  auto fits = [&](int64_t us) { return R * tile_row_stride_bound(C * us, us) <= budget; };
+  if (fits(U)) return U;
+  const int64_t align = pow2_factor(U, 16);
+  int64_t best = 0;
+  for (int64_t q = 1; q * q <= U; ++q) {
+    if (U % q) continue;
+    for (int64_t us : {q, U / q})
+      if (us % align == 0 && us >= min_seg && us > best && fits(us)) best = us;
+  }
+  return best;
+}
+
 struct CatPlan {
   bool ok = false;
   int64_t m, n, c, a, b;
@@ -304,7 +318,22 @@ std::vector<int64_t> divisors(int64_t x) {
   return d;
 }
 
-enum How { HOW_NONE = 0, HOW_TILE = 1, HOW_CAT = 2, HOW_SPLIT_R = 3, HOW_SPLIT_C = 4 };
+enum How {
+  HOW_NONE = 0,
+  HOW_TILE = 1,
+  HOW_CAT = 2,
+  HOW_SPLIT_R = 3,
+  HOW_SPLIT_C = 4,
+  HOW_TILE_STAGED = 5
+};
+
+// Relative time of one pass over the tensor, by pass class (contiguous LDS
+// tile = 1).  Sliced tiles move short segments; staged passes cross HBM
+// twice; Catanzaro passes evaluate heavier index maps.
+constexpr double kCostTile = 1.0;
+constexpr double kCostTileSliced = 1.3;
+constexpr double kCostTileStaged = 2.05;
+constexpr double kCostSplit = 0.001;
 
 struct Choice {
   int how = HOW_NONE;
@@ -329,28 +358,38 @@ struct Lowerer {
     auto key = std::make_tuple(R, C, U, depth);
     auto it = memo.find(key);
     if (it != memo.end()) return it->second;
-    if (pick_slice(R * C, U, prof.lds_max, prof.min_segment)) {
+    if (int64_t us = pick_tile_slice(R, C, U, prof.lds_max, prof.min_segment)) {
       ch.how = HOW_TILE;
-      ch.cost = 1.0;
-      memo[key] = ch;
-      return ch;
+      ch.cost = us == U ? kCostTile : kCostTileSliced;
+      if (us == U) {
+        memo[key] = ch;
+        return ch;
+      }
     }
     CatPlan cp = plan_catanzaro(R, C, U, prof, scratch_budget);
-    if (cp.ok) {
+    if (cp.ok && cp.cost < ch.cost) {
       ch.how = HOW_CAT;
       ch.cost = cp.cost;
     }
-    if (depth < 3 && ch.cost > 2.0 + 1e-9) {
+    // whole matrices (or slices) through scratch: copy out, tiled transpose back
+    if (ch.cost > kCostTileStaged) {
+      const int64_t us = pick_slice(R * C, U, scratch_budget, std::min<int64_t>(U, 256));
+      if (us) {
+        ch.how = HOW_TILE_STAGED;
+        ch.cost = kCostTileStaged;
+      }
+    }
+    if (depth < 3 && ch.cost > 2 * kCostTile + 1e-9) {
       // R = R1 * R0: (B*R1, R0, C, U) then (B, R1, C, R0*U)
       for (int64_t r0 : divisors(R)) {
         if (r0 <= 1 || r0 >= R) continue;
         double c1 = best(r0, C, U, depth + 1).cost;
         if (c1 >= ch.cost) continue;
         double c2 = best(R / r0, C, r0 * U, depth + 1).cost;
-        if (c1 + c2 + 1e-3 < ch.cost) {
+        if (c1 + c2 + kCostSplit < ch.cost) {
           ch.how = HOW_SPLIT_R;
           ch.f = r0;
-          ch.cost = c1 + c2 + 1e-3;
+          ch.cost = c1 + c2 + kCostSplit;
         }
       }
       // C = C1 * C0: (B, R, C1, C0*U) then (B*C1, R, C0, U)
@@ -359,10 +398,10 @@ struct Lowerer {
         double c2 = best(R, c0, U, depth + 1).cost;
         if (c2 >= ch.cost) continue;
         double c1 = best(R, C / c0, c0 * U, depth + 1).cost;
-        if (c1 + c2 + 1e-3 < ch.cost) {
+        if (c1 + c2 + kCostSplit < ch.cost) {
           ch.how = HOW_SPLIT_C;
           ch.f = c0;
-          ch.cost = c1 + c2 + 1e-3;
+          ch.cost = c1 + c2 + kCostSplit;
         }
       }
     }
@@ -384,7 +423,21 @@ struct Lowerer {
         op.R = R;
         op.C = C;
         op.U = U;
-        op.us = pick_slice(R * C, U, prof.lds_max, prof.min_segment);
+        op.us = pick_tile_slice(R, C, U, prof.lds_max, prof.min_segment);
+        ops.push_back(op);
+        return;
+      }
+      case HOW_TILE_STAGED: {
+        Op op;
+        op.kind = OP_TILE;
+        op.staged = 1;
+        op.stage = stage;
+        op.base = base;
+        op.B = B;
+        op.R = R;
+        op.C = C;
+        op.U = U;
+        op.us = pick_slice(R * C, U, scratch_budget, std::min<int64_t>(U, 256));
         ops.push_back(op);
         return;
       }

@@ -122,6 +122,28 @@ int estimate_native(const int64_t* dims, const int* perm1, int rank, size_t elem
                     const DeviceProfile& prof, int* n_sub, double* alpha_out,
                     size_t* extra_bytes);
 
+// ---- LDS layout of a tile footprint -------------------------------------------
+// Tile rows are padded so that column reads hit distinct LDS banks: the row
+// stride is an odd multiple of max(4, word) bytes.  The planner sizes tile
+// footprints with the largest word a slice allows, which bounds the stride
+// the kernel picks for any smaller word.
+inline int64_t tile_row_stride(int64_t rowbytes, int64_t wbytes) {
+  const int64_t g = wbytes > 4 ? wbytes : 4;
+  int64_t rs = (rowbytes + g - 1) / g * g;
+  if (((rs / g) & 1) == 0) rs += g;
+  return rs;
+}
+inline int64_t tile_row_stride_bound(int64_t rowbytes, int64_t us) {
+  int64_t w = 1;
+  while (w < 16 && us % (w * 2) == 0) w *= 2;
+  int64_t best = 0;
+  for (int64_t g = 1; g <= w; g *= 2) {
+    const int64_t r = tile_row_stride(rowbytes, g);
+    if (r > best) best = r;
+  }
+  return best;
+}
+
 // ---- integer helpers ----------------------------------------------------------
 int64_t gcd64(int64_t a, int64_t b);
 int64_t modinv64(int64_t a, int64_t m);  // a^{-1} mod m (0 when m == 1)
