@@ -71,6 +71,235 @@ def cpu_baseline(threads):
                       "oracle or_permute_mt (32x32-tiled, pthreads)"}
 
 
+def _carve(torch, specs, align=256):
+    offs, total = [], 0
+    for dims, _, elem in specs:
+        offs.append(total)
+        n = 1
+        for d in dims:
+            n *= d
+        total += (n * elem + align - 1) // align * align
+    return torch.empty(max(total, 1), dtype=torch.uint8, device="cuda:0"), offs
+
+
+def _vol(dims):
+    n = 1
+    for d in dims:
+        n *= d
+    return n
+
+
+def run_single(name, dims, perm, elem, alpha, mode, seed, steps):
+    """One large tensor: forward transposes (timed per step with pass times),
+    a sampled closed-form check of the first forward result, and a round trip
+    (forward then inverse) checked on the device against the pattern."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import eithot_amd as E
+    from oracle import pyoracle as O
+    vol = _vol(dims)
+    nbytes = vol * elem
+    buf = torch.empty(nbytes, dtype=torch.uint8, device="cuda:0")
+    E.fill_pattern(buf, vol, elem, seed, mode)
+    torch.cuda.synchronize()
+    inv_dims, inv_perm = inverse(dims, perm)
+    E.lib().eithot_set_pass_timing(1)
+    times, passes, peak = [], [], 0
+    for k in range(steps):
+        d, p = (dims, perm) if k % 2 == 0 else (inv_dims, inv_perm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        E.transpose_tensor(buf, d, p, elem, 1, alpha)
+        times.append(time.perf_counter() - t0)
+        n = E.lib().eithot_last_pass_times(None, 0)
+        arr = (ctypes.c_float * n)()
+        E.lib().eithot_last_pass_times(arr, n)
+        passes.append((k % 2, list(arr)))
+        peak = max(peak, E.last_peak_extra_bytes())
+        print(f"bench {name}: step {k} {times[-1] * 1e3:.2f} ms", file=sys.stderr, flush=True)
+        if k == 0:
+            # sampled closed-form check of the forward result
+            rng = np.random.default_rng(7)
+            src = rng.integers(0, vol, size=20000, dtype=np.int64)
+            dst = np.array([O.dest_index(int(i), dims, [q - 1 for q in perm]) for i in src])
+            idx = torch.from_numpy(dst).to("cuda:0")
+            words = {1: torch.uint8, 2: torch.int16, 4: torch.int32, 8: torch.int64}[elem]
+            got = buf.view(words)[idx].cpu().numpy().view({1: np.uint8, 2: np.uint16, 4: np.uint32,
+                                                           8: np.uint64}[elem])
+            mask = (1 << (8 * elem)) - 1
+            want = np.array([O.value(seed, int(i), mode) & mask for i in src], dtype=np.uint64)
+            if not np.array_equal(got.astype(np.uint64), want):
+                print(f"bench {name}: sampled check failed", file=sys.stderr)
+                sys.exit(1)
+    E.lib().eithot_set_pass_timing(0)
+    if steps % 2:
+        E.transpose_tensor(buf, inv_dims, inv_perm, elem, 1, alpha)
+    mism = E.verify_pattern(buf, dims, list(range(1, len(dims) + 1)), elem, seed, mode)
+    if mism:
+        print(f"bench {name}: round trip differs in {mism} elements", file=sys.stderr)
+        sys.exit(1)
+    fwd = [t for k, t in enumerate(times) if k % 2 == 0]
+    t = sorted(fwd)[len(fwd) // 2]
+    ops, _ = E.plan_describe(dims, perm, elem, 1, alpha)
+    fwd_pass = [p for w, p in passes if w == 0]
+    per_pass = []
+    for i, op in enumerate(ops):
+        ms = sorted(x[i] for x in fwd_pass)[len(fwd_pass) // 2]
+        per_pass.append({"kernel": kernel_name(op), "ms": round(ms, 4),
+                         "GBps_2x_tensor": round(2 * nbytes / (ms / 1e3) / 1e9, 1),
+                         "B": op["B"], "R": op["R"], "C": op["C"], "U": op["U"], "us": op["us"]})
+    return {"workload": name, "dims": dims, "perm": perm, "elem_bytes": elem, "alpha": alpha,
+            "tensor_bytes": nbytes, "ms": round(t * 1e3, 3),
+            "effective_GBps": round(2 * nbytes / t / 1e9, 2),
+            "hbm_frac_effective": round(2 * nbytes / t / 1e9 / HBM_PEAK_GBS, 4),
+            "peak_extra_bytes": peak, "peak_extra_frac": round(peak / nbytes, 5),
+            "passes": per_pass, "checks": "sampled closed form (20000 elements) + device round trip"}
+
+
+def run_batch(count=4096, reps=5):
+    import numpy as np
+    import torch
+
+    import eithot_amd as E
+    from eithot_amd.workloads import batch_specs
+    specs = batch_specs(count=count)
+    buf, offs = _carve(torch, specs)
+    base = buf.data_ptr()
+    total = sum(_vol(d) * e for d, _, e in specs)
+
+    def refill():
+        for k, ((dims, _, elem), off) in enumerate(zip(specs, offs)):
+            E.fill_pattern(base + off, _vol(dims), elem, 3000 + k, 1)
+        torch.cuda.synchronize()
+
+    def check(sample):
+        for k in sample:
+            dims, perm, elem = specs[k]
+            if E.verify_pattern(base + offs[k], dims, perm, elem, 3000 + k, 1):
+                print(f"bench batch: tensor {k} differs", file=sys.stderr)
+                sys.exit(1)
+
+    sample = list(range(0, count, max(1, count // 64)))
+    refill()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for (dims, perm, elem), off in zip(specs, offs):
+        E.transpose_ptr(base + off, dims, perm, elem, 1, 0.05)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    check(sample)
+    print(f"bench batch: plain loop {t_plain * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    refill()
+    b = E.Batch()
+    for (dims, perm, elem), off in zip(specs, offs):
+        b.add(base + off, dims, perm, elem)
+    tb = time.perf_counter()
+    b.build(alpha=0.05, tune=True)
+    t_build = time.perf_counter() - tb
+    print(f"bench batch: built + tuned in {t_build:.2f} s", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run()
+    t_first = time.perf_counter() - t0
+    check(sample)
+    reps_t = []
+    for _ in range(reps):  # replays permute the bytes again: same traffic, timing only
+        t0 = time.perf_counter()
+        b.run()
+        reps_t.append(time.perf_counter() - t0)
+    t_batch = sorted(reps_t + [t_first])[(reps + 1) // 2]
+    info = b.info()
+    b.close()
+    return {"workload": "batched: 4096 tensors, orders 2-6, dims log-uniform [2,512], <= 4Mi "
+                        "elements each, 2/4/8-byte elements, seed 3",
+            "tensors": count, "batch_bytes": total,
+            "plain_loop_ms": round(t_plain * 1e3, 2),
+            "plain_tensors_per_s": round(count / t_plain, 1),
+            "batched_ms": round(t_batch * 1e3, 2),
+            "batched_tensors_per_s": round(count / t_batch, 1),
+            "batched_GBps": round(2 * total / t_batch / 1e9, 1),
+            "speedup_vs_plain": round(t_plain / t_batch, 2), "launches": info["launches"],
+            "graph": info["graph"], "scratch_bytes": info["scratch_bytes"],
+            "build_and_tune_s": round(t_build, 2),
+            "checks": f"{len(sample)} tensors verified on device after the plain loop and the batch"}
+
+
+def run_serve(count=96, rates=(100, 300, 1000, 3000)):
+    import numpy as np
+    import torch
+
+    import eithot_amd as E
+    from eithot_amd.workloads import arrivals, serve_specs
+    specs = serve_specs(count=count)
+    buf, offs = _carve(torch, specs)
+    base = buf.data_ptr()
+    resident = sum(_vol(d) * e for d, _, e in specs)
+    pool = int(0.05 * resident)
+
+    def refill():
+        for k, ((dims, _, elem), off) in enumerate(zip(specs, offs)):
+            E.fill_pattern(base + off, _vol(dims), elem, 4000 + k, 1)
+        torch.cuda.synchronize()
+
+    def verify():
+        bad = 0
+        for k in range(count):
+            dims, perm, elem = specs[k]
+            bad += E.verify_pattern(base + offs[k], dims, perm, elem, 4000 + k, 1) != 0
+        return bad
+
+    results = []
+    for rate in rates:
+        arr = arrivals(4, rate, count)
+        refill()
+        srv = E.Server(pool, max_inflight=16, nstreams=4)
+        t0 = time.perf_counter()
+        ids = []
+        i = 0
+        while i < count:
+            now = time.perf_counter() - t0
+            if now >= arr[i]:
+                ids.append(srv.submit(base + offs[i], *specs[i]))
+                i += 1
+            else:
+                srv.poll(False)
+        srv.drain()
+        makespan = time.perf_counter() - t0
+        lat = np.array([srv.status(r)[1] for r in ids])
+        srv.close()
+        bad = verify()
+        # FIFO plain-method queue on the same arrivals
+        refill()
+        t0 = time.perf_counter()
+        flat = []
+        for k in range(count):
+            while time.perf_counter() - t0 < arr[k]:
+                pass
+            E.transpose_ptr(base + offs[k], *specs[k], 1, 0.05)
+            flat.append((time.perf_counter() - t0 - arr[k]) * 1e3)
+        fmake = time.perf_counter() - t0
+        flat = np.array(flat)
+        moved = sum(2 * _vol(d) * e for d, _, e in specs)
+        results.append({"rate_per_s": rate,
+                        "served": {"tensors_per_s": round(count / makespan, 1),
+                                   "GBps": round(moved / makespan / 1e9, 1),
+                                   "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                                   "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                                   "mismatched_tensors": int(bad)},
+                        "fifo_plain": {"tensors_per_s": round(count / fmake, 1),
+                                       "GBps": round(moved / fmake / 1e9, 1),
+                                       "p50_ms": round(float(np.percentile(flat, 50)), 3),
+                                       "p99_ms": round(float(np.percentile(flat, 99)), 3)}})
+        print(f"bench serve: rate {rate}/s done", file=sys.stderr, flush=True)
+    return {"workload": "served: Poisson arrivals, tensors 1 MB-4 GB log-uniform, orders 3-6, "
+                        "1/2/4/8-byte elements, seed 4; scratch pool 5% of resident bytes",
+            "requests": count, "resident_bytes": resident, "pool_bytes": pool,
+            "max_inflight": 16, "streams": 4, "sweep": results}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +307,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="c2",
+                    choices=["c2", "c1", "c3", "batch", "serve"],
+                    help="c2 (default) prints the benchmark line; the others print their own "
+                         "JSON report")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the path runs on one device; "
@@ -85,6 +318,20 @@ def main():
         sys.exit(2)
     if args.steps % 2:
         args.steps += 1  # forward + inverse pairs
+    if args.workload == "c1":
+        print(json.dumps(run_single("c1: (2,4096,1024,2) perm 1 3 2 4, uint32 v(i)=i",
+                                    [2, 4096, 1024, 2], [1, 3, 2, 4], 4, 0.05, 0, 0, 20)))
+        return
+    if args.workload == "c3":
+        print(json.dumps(run_single("c3: (2,163840,65536,2) perm 1 3 2 4, uint32 seed 2, 172 GB",
+                                    [2, 163840, 65536, 2], [1, 3, 2, 4], 4, 0.05, 1, 2, 2)))
+        return
+    if args.workload == "batch":
+        print(json.dumps(run_batch()))
+        return
+    if args.workload == "serve":
+        print(json.dumps(run_serve()))
+        return
 
     import torch
 

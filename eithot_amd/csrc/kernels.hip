@@ -17,6 +17,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 #include "eithot_status.h"
 #include "maps.hpp"
@@ -135,10 +136,7 @@ __device__ __forceinline__ Idx row_src(const Args<Idx>& a, Idx i, Idx j) {
 // LDS: footprint k row i at k*fpl + i*rs (rs padded), cell j at +j*us
 // ---------------------------------------------------------------------------
 template <typename Word, typename Idx, int VEC>
-__global__ __launch_bounds__(512) void k_tile(Args<Idx> a, int remap) {
-  extern __shared__ __align__(16) unsigned char lds[];
-  const Idx wg = (Idx)xcd_logical(blockIdx.x, gridDim.x, remap);
-  const Idx f0 = wg * a.K;
+__device__ __forceinline__ void tile_body(const Args<Idx>& a, const Idx f0, unsigned char* lds) {
   const Idx nf = (a.nfp - f0) < a.K ? (a.nfp - f0) : a.K;
   const Idx tid = threadIdx.x, nt = blockDim.x;
   constexpr Idx w = sizeof(Word);
@@ -287,10 +285,7 @@ __global__ __launch_bounds__(512) void k_tile(Args<Idx> a, int remap) {
 // the panel's segment (W*us bytes) at k*fpl + i*rs
 // ---------------------------------------------------------------------------
 template <typename Word, typename Idx, int MAP, int VEC>
-__global__ __launch_bounds__(512) void k_col(Args<Idx> a, int remap) {
-  extern __shared__ __align__(16) unsigned char lds[];
-  const Idx wg = (Idx)xcd_logical(blockIdx.x, gridDim.x, remap);
-  const Idx f0 = wg * a.K;
+__device__ __forceinline__ void col_body(const Args<Idx>& a, const Idx f0, unsigned char* lds) {
   const Idx nf = (a.nfp - f0) < a.K ? (a.nfp - f0) : a.K;
   const Idx tid = threadIdx.x, nt = blockDim.x;
   constexpr Idx w = sizeof(Word);
@@ -379,10 +374,7 @@ __global__ __launch_bounds__(512) void k_col(Args<Idx> a, int remap) {
 // row pass: K rows of C cells per workgroup; LDS row k at k*rs
 // ---------------------------------------------------------------------------
 template <typename Word, typename Idx, int MAP, int VEC>
-__global__ __launch_bounds__(512) void k_row(Args<Idx> a, int remap) {
-  extern __shared__ __align__(16) unsigned char lds[];
-  const Idx wg = (Idx)xcd_logical(blockIdx.x, gridDim.x, remap);
-  const Idx f0 = wg * a.K;
+__device__ __forceinline__ void row_body(const Args<Idx>& a, const Idx f0, unsigned char* lds) {
   const Idx nf = (a.nfp - f0) < a.K ? (a.nfp - f0) : a.K;
   const Idx tid = threadIdx.x, nt = blockDim.x;
   constexpr Idx w = sizeof(Word);
@@ -457,6 +449,23 @@ __global__ __launch_bounds__(512) void k_row(Args<Idx> a, int remap) {
           reinterpret_cast<const Word*>(lds)[k * a.st_per + s * a.wu + x];
     }
   }
+}
+
+// one workgroup per group of K footprints
+template <typename Word, typename Idx, int VEC>
+__global__ __launch_bounds__(512) void k_tile(Args<Idx> a, int remap) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  tile_body<Word, Idx, VEC>(a, (Idx)xcd_logical(blockIdx.x, gridDim.x, remap) * a.K, lds);
+}
+template <typename Word, typename Idx, int MAP, int VEC>
+__global__ __launch_bounds__(512) void k_col(Args<Idx> a, int remap) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  col_body<Word, Idx, MAP, VEC>(a, (Idx)xcd_logical(blockIdx.x, gridDim.x, remap) * a.K, lds);
+}
+template <typename Word, typename Idx, int MAP, int VEC>
+__global__ __launch_bounds__(512) void k_row(Args<Idx> a, int remap) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  row_body<Word, Idx, MAP, VEC>(a, (Idx)xcd_logical(blockIdx.x, gridDim.x, remap) * a.K, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -684,101 +693,153 @@ int64_t pick_K(int64_t nfp, int64_t fpb, int64_t lds_target) {
   return K;
 }
 
-template <typename Word, typename Idx, int VEC>
-hipError_t launch_tile(Args<Idx> a, const Op& op, const LaunchCfg& cfg, hipStream_t st) {
-  const int64_t w = sizeof(Word);
-  const int64_t rowbytes = op.C * op.us;
-  a.rs = (Idx)tile_row_stride(rowbytes, w);
-  a.fpl = (Idx)(op.R * a.rs);
-  const bool chunk16 = a.S == 1 && rowbytes % 16 == 0 &&
-                       ((uintptr_t)(a.data + a.base) % 16) == 0 && (a.rs % 4) == 0;
-  a.cpr = chunk16 ? (Idx)(rowbytes / 16) : 0;
-  a.fcpr.init(a.cpr ? a.cpr : 1);
-  const bool cells16 = !chunk16 && VEC == 1 && w == 16 && op.us % 16 == 0 && op.U % 16 == 0 &&
-                       ((uintptr_t)(a.data + a.base) % 16) == 0 && a.rs % 16 == 0;
-  a.cps = cells16 ? (Idx)(op.us / 16) : 0;
+// Kernel variant of an LDS pass: kind/map (0 tile, 1..4 column maps, 5..6
+// row maps), log2 of the word size, and whether cells are packed into
+// 16-byte vector stores.
+inline int kindmap_of(const Op& op) {
+  if (op.kind == OP_TILE) return 0;
+  if (op.kind == OP_COL) return op.map;  // MAP_ROT..MAP_COLSHINV = 1..4
+  return op.map;                         // MAP_ROWG, MAP_ROWD = 5, 6
+}
+
+struct Prepared {
+  int variant = -1;  // kindmap * 10 + log2(word) * 2 + vec
+  int idx64 = 0;
+  int64_t K = 1;
+  int64_t lds = 0;   // LDS bytes per workgroup
+  int threads = 256;
+  int64_t groups = 0;
+};
+
+// Fills every LDS-kernel field of `a` for op (not staged) under cfg.
+template <typename Idx>
+void prepare_lds(const Op& op, char* data, const LaunchCfg& cfg, Args<Idx>& a, Prepared& p) {
+  const int64_t w = word_bytes(op, (uintptr_t)data);
+  a = make_args<Idx>(op, data, w);
+  int wl = 0;
+  while ((1 << wl) < w) ++wl;
+  const int64_t V = w < 16 ? 16 / w : 1;
+  const bool one_word = a.wu == 1 && a.S == 1;
+  const uintptr_t base = (uintptr_t)(data + op.base);
+  const int km = kindmap_of(op);
+  bool vec = false;
+  if (op.kind == OP_TILE) {
+    vec = V > 1 && one_word && op.R % V == 0 && base % 16 == 0;
+    const int64_t rowbytes = op.C * op.us;
+    a.rs = (Idx)tile_row_stride(rowbytes, w);
+    a.fpl = (Idx)(op.R * a.rs);
+    const bool chunk16 = a.S == 1 && rowbytes % 16 == 0 && base % 16 == 0 && (a.rs % 4) == 0;
+    a.cpr = chunk16 ? (Idx)(rowbytes / 16) : 0;
+    const bool cells16 = !chunk16 && !vec && w == 16 && op.us % 16 == 0 && op.U % 16 == 0 &&
+                         base % 16 == 0 && a.rs % 16 == 0;
+    a.cps = cells16 ? (Idx)(op.us / 16) : 0;
+    a.ld_per = a.cps ? a.R * a.C * a.cps : a.R * a.C * a.wu;
+    a.vpr = vec ? (Idx)(op.R / V) : 1;
+    a.st_per = vec ? (Idx)(op.C * a.vpr) : a.R * a.C * a.wu;
+  } else if (op.kind == OP_COL) {
+    vec = V > 1 && one_word && op.W % V == 0 && op.C % op.W == 0 && base % 16 == 0 &&
+          (op.C * op.U) % 16 == 0;
+    const int64_t seg = op.W * op.us;
+    a.rs = (Idx)seg;
+    a.fpl = (Idx)(op.R * seg);
+    const bool chunk16 = op.C % op.W == 0 && seg % 16 == 0 && base % 16 == 0 && (a.CU % 16) == 0 &&
+                         (op.us == op.U ? true : (op.W == 1 && op.us % 16 == 0));
+    a.cpr = chunk16 ? (Idx)(seg / 16) : 0;
+    a.cps = 0;
+    a.ld_per = chunk16 ? a.R * a.cpr : a.R * a.W * a.wu;
+    a.vpr = vec ? (Idx)(op.W / V) : 1;
+    a.st_per = vec ? a.R * a.vpr : a.R * a.W * a.wu;
+  } else {
+    vec = V > 1 && one_word && op.C % V == 0 && base % 16 == 0;
+    a.rs = (Idx)(op.C * op.us);
+    a.fpl = a.rs;
+    const bool chunk16 = a.S == 1 && (a.rs % 16) == 0 && base % 16 == 0;
+    a.cpr = chunk16 ? (Idx)(a.rs / 16) : 0;
+    a.cps = 0;
+    a.ld_per = a.C * a.wu;
+    a.vpr = vec ? (Idx)(op.C / V) : 1;
+    a.st_per = vec ? a.vpr : a.C * a.wu;
+  }
   a.cps_log2 = -1;
   if (a.cps && (a.cps & (a.cps - 1)) == 0) {
     a.cps_log2 = 0;
     while (((Idx)1 << a.cps_log2) < a.cps) ++a.cps_log2;
   }
-  a.fcps.init(a.cps ? a.cps : 1);
-  a.ld_per = a.cps ? a.R * a.C * a.cps : a.R * a.C * a.wu;
-  a.fld.init(a.ld_per);
-  if (VEC > 1) {
-    a.vpr = (Idx)(op.R / VEC);
-    a.st_per = (Idx)(op.C * a.vpr);
-  } else {
-    a.vpr = 1;
-    a.st_per = a.R * a.C * a.wu;
-  }
-  a.fst.init(a.st_per);
-  a.fvpr.init(a.vpr);
-  const int64_t K = pick_K((int64_t)a.nfp, (int64_t)a.fpl, cfg.lds_target);
-  a.K = (Idx)K;
-  const int64_t nwg = ((int64_t)a.nfp + K - 1) / K;
-  const int threads = K * a.fpl > 96 * 1024 ? cfg.threads_big : cfg.threads;
-  hipLaunchKernelGGL((k_tile<Word, Idx, VEC>), dim3((uint32_t)nwg), dim3(threads),
-                     (size_t)(K * a.fpl), st, a, cfg.xcd_remap);
-  return hipGetLastError();
-}
-
-template <typename Word, typename Idx, int MAP, int VEC>
-hipError_t launch_col(Args<Idx> a, const Op& op, const LaunchCfg& cfg, hipStream_t st) {
-  const int64_t seg = op.W * op.us;
-  a.rs = (Idx)seg;
-  a.fpl = (Idx)(op.R * seg);
-  const bool full = op.C % op.W == 0;
-  const bool chunk16 = full && seg % 16 == 0 && ((uintptr_t)(a.data + a.base) % 16) == 0 &&
-                       (a.CU % 16) == 0 && (op.us == op.U || op.us % 16 == 0) &&
-                       ((op.us == op.U) ? true : op.W == 1);
-  a.cpr = chunk16 ? (Idx)(seg / 16) : 0;
   a.fcpr.init(a.cpr ? a.cpr : 1);
-  a.ld_per = chunk16 ? a.R * a.cpr : a.R * a.W * a.wu;
+  a.fcps.init(a.cps ? a.cps : 1);
   a.fld.init(a.ld_per);
-  if (VEC > 1) {
-    a.vpr = (Idx)(op.W / VEC);
-    a.st_per = a.R * a.vpr;
-  } else {
-    a.vpr = 1;
-    a.st_per = a.R * a.W * a.wu;
-  }
   a.fst.init(a.st_per);
   a.fvpr.init(a.vpr);
-  const int64_t K = pick_K((int64_t)a.nfp, (int64_t)a.fpl, cfg.lds_target);
-  a.K = (Idx)K;
-  const int64_t nwg = ((int64_t)a.nfp + K - 1) / K;
-  const int threads = K * a.fpl > 96 * 1024 ? cfg.threads_big : cfg.threads;
-  hipLaunchKernelGGL((k_col<Word, Idx, MAP, VEC>), dim3((uint32_t)nwg), dim3(threads),
-                     (size_t)(K * a.fpl), st, a, cfg.xcd_remap);
-  return hipGetLastError();
+  p.K = pick_K((int64_t)a.nfp, (int64_t)a.fpl, cfg.lds_target);
+  a.K = (Idx)p.K;
+  p.lds = p.K * (int64_t)a.fpl;
+  p.threads = p.lds > 96 * 1024 ? cfg.threads_big : cfg.threads;
+  p.groups = ((int64_t)a.nfp + p.K - 1) / p.K;
+  p.variant = km * 10 + wl * 2 + (vec ? 1 : 0);
+  p.idx64 = sizeof(Idx) == 8;
 }
 
-template <typename Word, typename Idx, int MAP, int VEC>
-hipError_t launch_row(Args<Idx> a, const Op& op, const LaunchCfg& cfg, hipStream_t st) {
-  a.rs = (Idx)(op.C * op.us);
-  a.fpl = a.rs;
-  const bool chunk16 = a.S == 1 && (a.rs % 16) == 0 && ((uintptr_t)(a.data + a.base) % 16) == 0;
-  a.cpr = chunk16 ? (Idx)(a.rs / 16) : 0;
-  a.ld_per = a.C * a.wu;
-  a.fld.init(a.ld_per);
-  if (VEC > 1) {
-    a.vpr = (Idx)(op.C / VEC);
-    a.st_per = a.vpr;
-  } else {
-    a.vpr = 1;
-    a.st_per = a.C * a.wu;
-  }
-  a.fst.init(a.st_per);
-  a.fvpr.init(a.vpr);
-  const int64_t K = pick_K((int64_t)a.nfp, (int64_t)a.fpl, cfg.lds_target);
-  a.K = (Idx)K;
-  const int64_t nwg = ((int64_t)a.nfp + K - 1) / K;
-  const int threads = K * a.fpl > 96 * 1024 ? cfg.threads_big : cfg.threads;
-  hipLaunchKernelGGL((k_row<Word, Idx, MAP, VEC>), dim3((uint32_t)nwg), dim3(threads),
-                     (size_t)(K * a.fpl), st, a, cfg.xcd_remap);
-  return hipGetLastError();
+// calls F::template run<Word, KINDMAP, VEC>() for a variant id
+template <int KM, int WL, int VF, typename F>
+hipError_t with_variant3(F& f) {
+  using Word = typename std::conditional<
+      WL == 4, W16,
+      typename std::conditional<
+          WL == 3, uint64_t,
+          typename std::conditional<WL == 2, uint32_t,
+                                    typename std::conditional<WL == 1, uint16_t, uint8_t>::type>::
+              type>::type>::type;
+  constexpr int V = (16 >> WL) > 1 ? (16 >> WL) : 1;
+  return f.template run<Word, KM, VF ? V : 1>();
 }
+template <int KM, int WL, typename F>
+hipError_t with_variant2(int vf, F& f) {
+  return vf ? with_variant3<KM, WL, (WL < 4 ? 1 : 0)>(f) : with_variant3<KM, WL, 0>(f);
+}
+template <int KM, typename F>
+hipError_t with_variant1(int wl, int vf, F& f) {
+  switch (wl) {
+    case 0: return with_variant2<KM, 0>(vf, f);
+    case 1: return with_variant2<KM, 1>(vf, f);
+    case 2: return with_variant2<KM, 2>(vf, f);
+    case 3: return with_variant2<KM, 3>(vf, f);
+    default: return with_variant2<KM, 4>(vf, f);
+  }
+}
+template <typename F>
+hipError_t with_variant(int variant, F& f) {
+  const int km = variant / 10, wl = (variant % 10) / 2, vf = variant % 2;
+  switch (km) {
+    case 0: return with_variant1<0>(wl, vf, f);
+    case 1: return with_variant1<1>(wl, vf, f);
+    case 2: return with_variant1<2>(wl, vf, f);
+    case 3: return with_variant1<3>(wl, vf, f);
+    case 4: return with_variant1<4>(wl, vf, f);
+    case 5: return with_variant1<5>(wl, vf, f);
+    default: return with_variant1<6>(wl, vf, f);
+  }
+}
+
+template <typename Idx>
+struct LdsLauncher {
+  const Args<Idx>& a;
+  const Prepared& p;
+  const LaunchCfg& cfg;
+  hipStream_t st;
+  template <typename Word, int KM, int V>
+  hipError_t run() {
+    const dim3 grid((uint32_t)p.groups), block(p.threads);
+    if (KM == 0)
+      hipLaunchKernelGGL((k_tile<Word, Idx, V>), grid, block, (size_t)p.lds, st, a, cfg.xcd_remap);
+    else if (KM <= 4)
+      hipLaunchKernelGGL((k_col<Word, Idx, (KM <= 4 ? KM : 1), V>), grid, block, (size_t)p.lds, st, a,
+                         cfg.xcd_remap);
+    else
+      hipLaunchKernelGGL((k_row<Word, Idx, (KM >= 5 ? KM : 5), V>), grid, block, (size_t)p.lds, st, a,
+                         cfg.xcd_remap);
+    return hipGetLastError();
+  }
+};
 
 template <typename Word, typename Idx, int KIND, int MAP>
 hipError_t launch_staged(Args<Idx> a, const Op& op, char* scratch, int64_t scratch_bytes,
@@ -840,52 +901,97 @@ hipError_t launch_staged(Args<Idx> a, const Op& op, char* scratch, int64_t scrat
   return hipSuccess;
 }
 
-template <typename Word, typename Idx, int KIND, int MAP>
-hipError_t run_kind(const Op& op, char* d, char* s, int64_t sb, const LaunchCfg& c, hipStream_t st) {
-  Args<Idx> a = make_args<Idx>(op, d, sizeof(Word));
-  if (op.staged) return launch_staged<Word, Idx, KIND, MAP>(a, op, s, sb, st);
-  constexpr int V = 16 / sizeof(Word) > 1 ? 16 / sizeof(Word) : 1;
-  const bool one_word = a.wu == 1 && a.S == 1;
-  const uintptr_t base = (uintptr_t)(d + op.base);
-  if (KIND == OP_TILE) {
-    if (V > 1 && one_word && op.R % V == 0 && base % 16 == 0)
-      return launch_tile<Word, Idx, V>(a, op, c, st);
-    return launch_tile<Word, Idx, 1>(a, op, c, st);
+template <typename Idx>
+struct StagedLauncher {
+  const Op& op;
+  Args<Idx>& a;
+  char* s;
+  int64_t sb;
+  hipStream_t st;
+  template <typename Word, int KM, int V>
+  hipError_t run() {
+    if (KM == 0) return launch_staged<Word, Idx, OP_TILE, MAP_NONE>(a, op, s, sb, st);
+    if (KM <= 4) return launch_staged<Word, Idx, OP_COL, (KM <= 4 ? KM : 1)>(a, op, s, sb, st);
+    return launch_staged<Word, Idx, OP_ROW, (KM >= 5 ? KM : 5)>(a, op, s, sb, st);
   }
-  if (KIND == OP_COL) {
-    if (V > 1 && one_word && op.W % V == 0 && op.C % op.W == 0 && base % 16 == 0 &&
-        (op.C * op.U) % 16 == 0)
-      return launch_col<Word, Idx, MAP, V>(a, op, c, st);
-    return launch_col<Word, Idx, MAP, 1>(a, op, c, st);
+};
+
+template <typename Idx>
+hipError_t run_op(const Op& op, char* d, char* s, int64_t sb, const LaunchCfg& c, hipStream_t st) {
+  if (op.staged) {
+    const int64_t w = word_bytes(op, (uintptr_t)d);
+    Args<Idx> a = make_args<Idx>(op, d, w);
+    StagedLauncher<Idx> L{op, a, s, sb, st};
+    int wl = 0;
+    while ((1 << wl) < w) ++wl;
+    return with_variant(kindmap_of(op) * 10 + wl * 2, L);
   }
-  if (V > 1 && one_word && op.C % V == 0 && base % 16 == 0)
-    return launch_row<Word, Idx, MAP, V>(a, op, c, st);
-  return launch_row<Word, Idx, MAP, 1>(a, op, c, st);
+  Args<Idx> a;
+  Prepared p;
+  prepare_lds<Idx>(op, d, c, a, p);
+  LdsLauncher<Idx> L{a, p, c, st};
+  return with_variant(p.variant, L);
 }
 
-template <typename Word, typename Idx>
-hipError_t run_map(const Op& op, char* d, char* s, int64_t sb, const LaunchCfg& c, hipStream_t st) {
-  if (op.kind == OP_TILE) return run_kind<Word, Idx, OP_TILE, MAP_NONE>(op, d, s, sb, c, st);
-  if (op.kind == OP_COL) {
-    switch (op.map) {
-      case MAP_ROT: return run_kind<Word, Idx, OP_COL, MAP_ROT>(op, d, s, sb, c, st);
-      case MAP_ROTINV: return run_kind<Word, Idx, OP_COL, MAP_ROTINV>(op, d, s, sb, c, st);
-      case MAP_COLSH: return run_kind<Word, Idx, OP_COL, MAP_COLSH>(op, d, s, sb, c, st);
-      case MAP_COLSHINV: return run_kind<Word, Idx, OP_COL, MAP_COLSHINV>(op, d, s, sb, c, st);
+
+// ---------------------------------------------------------------------------
+// persistent batched pass: one launch runs one pass of many tensors; each
+// workgroup walks a table of (op, first footprint) items
+// ---------------------------------------------------------------------------
+struct BatchOpDev {
+  int variant;
+  int pad[3];
+  Args<uint32_t> a;
+};
+struct BatchItemDev {
+  uint32_t op;
+  uint32_t f0;
+};
+
+template <int KM, int WL, int VF>
+__device__ __forceinline__ void batch_body(const Args<uint32_t>& a, uint32_t f0, unsigned char* lds) {
+  using Word = typename std::conditional<
+      WL == 4, W16,
+      typename std::conditional<
+          WL == 3, uint64_t,
+          typename std::conditional<WL == 2, uint32_t,
+                                    typename std::conditional<WL == 1, uint16_t, uint8_t>::type>::
+              type>::type>::type;
+  constexpr int V = VF ? (16 >> WL) : 1;
+  if (KM == 0) tile_body<Word, uint32_t, V>(a, f0, lds);
+  else if (KM <= 4) col_body<Word, uint32_t, (KM <= 4 ? KM : 1), V>(a, f0, lds);
+  else row_body<Word, uint32_t, (KM >= 5 ? KM : 5), V>(a, f0, lds);
+}
+
+#define EH_CASE(KM, WL, VF) \
+  case KM * 10 + WL * 2 + VF: batch_body<KM, WL, VF>(op.a, item.f0, lds); break;
+#define EH_CASES_W(KM, WL) EH_CASE(KM, WL, 0) EH_CASE(KM, WL, 1)
+#define EH_CASES(KM) \
+  EH_CASES_W(KM, 0) EH_CASES_W(KM, 1) EH_CASES_W(KM, 2) EH_CASES_W(KM, 3) EH_CASE(KM, 4, 0)
+
+__global__ __launch_bounds__(512) void k_batch(const BatchOpDev* __restrict__ ops,
+                                               const BatchItemDev* __restrict__ items,
+                                               uint32_t nitems) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const BatchItemDev item = items[it];
+    const BatchOpDev& op = ops[item.op];
+    switch (op.variant) {
+      EH_CASES(0)
+      EH_CASES(1)
+      EH_CASES(2)
+      EH_CASES(3)
+      EH_CASES(4)
+      EH_CASES(5)
+      EH_CASES(6)
+      default: break;
     }
+    __syncthreads();  // the next item reuses the LDS
   }
-  if (op.kind == OP_ROW) {
-    if (op.map == MAP_ROWG) return run_kind<Word, Idx, OP_ROW, MAP_ROWG>(op, d, s, sb, c, st);
-    if (op.map == MAP_ROWD) return run_kind<Word, Idx, OP_ROW, MAP_ROWD>(op, d, s, sb, c, st);
-  }
-  return hipErrorInvalidValue;
 }
-
-template <typename Word>
-hipError_t run_idx(const Op& op, char* d, char* s, int64_t sb, const LaunchCfg& c, hipStream_t st) {
-  if (needs64(op)) return run_map<Word, uint64_t>(op, d, s, sb, c, st);
-  return run_map<Word, uint32_t>(op, d, s, sb, c, st);
-}
+#undef EH_CASES
+#undef EH_CASES_W
+#undef EH_CASE
 
 // ---- pattern fill / verify ----------------------------------------------------
 
@@ -948,23 +1054,100 @@ __global__ void k_verify(const T* p, VerifyArgs v, uint64_t seed_mixed, int mode
 
 int64_t op_lds_bytes(const Op& op, const LaunchCfg& cfg) {
   if (op.staged || op.kind == 0) return 0;
-  int64_t fpb;
-  if (op.kind == OP_TILE) fpb = op.R * tile_row_stride(op.C * op.us, 16);
-  else fpb = op.footprint_units() * op.us;
-  return pick_K(op.num_footprints(), fpb, cfg.lds_target) * fpb;
+  Args<uint64_t> a;
+  Prepared p;
+  prepare_lds<uint64_t>(op, nullptr, cfg, a, p);
+  return p.lds;
 }
 
 hipError_t launch_op(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
                      const LaunchCfg& cfg, hipStream_t stream) {
   if (op.kind == 0 || op.us <= 0 || op.U % op.us) return hipErrorInvalidValue;
-  switch (word_bytes(op, (uintptr_t)data)) {
-    case 16: return run_idx<W16>(op, data, scratch, scratch_bytes, cfg, stream);
-    case 8: return run_idx<uint64_t>(op, data, scratch, scratch_bytes, cfg, stream);
-    case 4: return run_idx<uint32_t>(op, data, scratch, scratch_bytes, cfg, stream);
-    case 2: return run_idx<uint16_t>(op, data, scratch, scratch_bytes, cfg, stream);
-    default: return run_idx<uint8_t>(op, data, scratch, scratch_bytes, cfg, stream);
-  }
+  if (needs64(op)) return run_op<uint64_t>(op, data, scratch, scratch_bytes, cfg, stream);
+  return run_op<uint32_t>(op, data, scratch, scratch_bytes, cfg, stream);
 }
+
+
+// ---------------------------------------------------------------------------
+// persistent batched launches (host side)
+// ---------------------------------------------------------------------------
+struct BatchLaunch {
+  BatchOpDev* d_ops = nullptr;
+  BatchItemDev* d_items = nullptr;
+  uint32_t nitems = 0;
+  uint32_t grid = 0;
+  int threads = 256;
+  int64_t lds = 0;
+};
+
+int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, const LaunchCfg& cfg,
+                       int grid_per_cu, int cus, BatchLaunch** out, std::vector<size_t>* rejected) {
+  *out = nullptr;
+  std::vector<BatchOpDev> hops;
+  std::vector<std::pair<int64_t, BatchItemDev>> items;
+  int64_t lds = 0;
+  bool big = false;
+  for (size_t i = 0; i < ops.size(); ++i) {
+    const Op& op = *ops[i].first;
+    if (op.staged || op.kind == 0 || needs64(op)) {
+      if (rejected) rejected->push_back(i);
+      continue;
+    }
+    BatchOpDev d{};
+    Prepared p;
+    prepare_lds<uint32_t>(op, ops[i].second, cfg, d.a, p);
+    d.variant = p.variant;
+    const uint32_t idx = (uint32_t)hops.size();
+    hops.push_back(d);
+    lds = std::max(lds, p.lds);
+    big = big || p.threads > cfg.threads;
+    for (int64_t g = 0; g < p.groups; ++g)
+      items.push_back({p.lds, BatchItemDev{idx, (uint32_t)(g * p.K)}});
+  }
+  if (items.empty()) return EITHOT_OK;
+  // largest items first so that round-robin assignment balances the tail
+  std::stable_sort(items.begin(), items.end(),
+                   [](const std::pair<int64_t, BatchItemDev>& x,
+                      const std::pair<int64_t, BatchItemDev>& y) { return x.first > y.first; });
+  std::vector<BatchItemDev> hitems(items.size());
+  for (size_t i = 0; i < items.size(); ++i) hitems[i] = items[i].second;
+  BatchLaunch* L = new BatchLaunch;
+  L->nitems = (uint32_t)hitems.size();
+  L->threads = big ? cfg.threads_big : cfg.threads;
+  L->lds = lds;
+  L->grid = grid_per_cu > 0 ? (uint32_t)std::min<int64_t>(L->nitems, (int64_t)cus * grid_per_cu)
+                            : L->nitems;
+  if (hipMalloc((void**)&L->d_ops, hops.size() * sizeof(BatchOpDev)) != hipSuccess ||
+      hipMalloc((void**)&L->d_items, hitems.size() * sizeof(BatchItemDev)) != hipSuccess) {
+    batch_launch_free(L);
+    return EITHOT_ERR_NOMEM;
+  }
+  if (hipMemcpy(L->d_ops, hops.data(), hops.size() * sizeof(BatchOpDev), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(L->d_items, hitems.data(), hitems.size() * sizeof(BatchItemDev),
+                hipMemcpyHostToDevice) != hipSuccess) {
+    batch_launch_free(L);
+    return EITHOT_ERR_DEVICE;
+  }
+  *out = L;
+  return EITHOT_OK;
+}
+
+hipError_t batch_launch_enqueue(const BatchLaunch* L, hipStream_t st) {
+  if (!L || !L->nitems) return hipSuccess;
+  hipLaunchKernelGGL(k_batch, dim3(L->grid), dim3(L->threads), (size_t)L->lds, st, L->d_ops,
+                     L->d_items, L->nitems);
+  return hipGetLastError();
+}
+
+void batch_launch_free(BatchLaunch* L) {
+  if (!L) return;
+  if (L->d_ops) (void)hipFree(L->d_ops);
+  if (L->d_items) (void)hipFree(L->d_items);
+  delete L;
+}
+
+int64_t batch_launch_items(const BatchLaunch* L) { return L ? L->nitems : 0; }
 
 hipError_t launch_fill(char* data, uint64_t n, int elem, uint64_t seed, int mode, uint64_t first,
                        hipStream_t stream) {
@@ -1045,6 +1228,11 @@ void cfg_word(int64_t bytes, hipError_t& e) {
 
 hipError_t configure_kernels(int64_t bytes) {
   hipError_t e = hipSuccess;
+  {
+    hipError_t r = hipFuncSetAttribute((const void*)k_batch,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (r != hipSuccess) e = r;
+  }
   cfg_word<W16>(bytes, e);
   cfg_word<uint64_t>(bytes, e);
   cfg_word<uint32_t>(bytes, e);

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <utility>
+#include <vector>
 
 #include "plan.hpp"
 
@@ -34,6 +36,18 @@ hipError_t launch_fill(char* data, uint64_t n, int elem, uint64_t seed, int mode
 hipError_t launch_verify(const char* data, const int64_t* dims, const int* perm0, int rank,
                          int elem, uint64_t seed, int mode, unsigned long long* d_mismatch,
                          hipStream_t stream);
+
+// Persistent batched launch: one pass (op) of many tensors in one kernel.
+// ops pairs each op with its tensor's device pointer; staged or 64-bit-index
+// ops are not batched and are returned in `rejected` (indices into ops).
+// grid_per_cu > 0 sizes a persistent grid (workgroups per CU), 0 launches
+// one workgroup per item.
+struct BatchLaunch;
+int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, const LaunchCfg& cfg,
+                       int grid_per_cu, int cus, BatchLaunch** out, std::vector<size_t>* rejected);
+hipError_t batch_launch_enqueue(const BatchLaunch* L, hipStream_t stream);
+void batch_launch_free(BatchLaunch* L);
+int64_t batch_launch_items(const BatchLaunch* L);
 
 // Raise the dynamic-LDS limit of every pass kernel to `bytes` (once per device).
 hipError_t configure_kernels(int64_t bytes);
