@@ -247,6 +247,18 @@ int64_t pick_tile_slice(int64_t R, int64_t C, int64_t U, int64_t budget, int64_t
   return best;
 }
 
+// Largest divisor of U whose `units` slices fit `cap` bytes, whatever its
+// alignment (narrow slices cost word width, but always exist).  0 if none.
+int64_t any_slice(int64_t units, int64_t U, int64_t cap) {
+  int64_t best = 0;
+  for (int64_t q = 1; q * q <= U; ++q) {
+    if (U % q) continue;
+    for (int64_t us : {q, U / q})
+      if (us > best && units * us <= cap) best = us;
+  }
+  return best;
+}
+
 struct CatPlan {
   bool ok = false;
   int64_t m, n, c, a, b;
@@ -269,15 +281,31 @@ CatPlan plan_catanzaro(int64_t R, int64_t C, int64_t U, const DeviceProfile& pro
   p.a = p.m / p.c;
   p.b = p.n / p.c;
   p.npass = p.c > 1 ? 3 : 2;
+  // A pass whose footprint (`units` whole units) fits neither LDS at an
+  // aligned slice: staged through scratch if one footprint fits the budget,
+  // else narrow slices in LDS, else narrow slices through scratch.
+  double narrow_cost = 0;
+  auto fallback = [&](int64_t units, int64_t& us, bool& staged) {
+    if (units * U <= scratch_budget) {
+      us = U;
+      staged = true;
+    } else if ((us = any_slice(units, U, prof.lds_max))) {
+      staged = false;
+      narrow_cost += 0.5;
+    } else if ((us = any_slice(units, U, scratch_budget))) {
+      staged = true;
+      narrow_cost += 0.5;
+    } else {
+      us = U;
+      staged = true;  // footprint exceeds the budget: rejected below
+    }
+  };
   // column passes
   p.col_staged = false;
   if (U >= 128) {
     p.col_W = 1;
     p.col_us = pick_slice(p.m, U, prof.lds_max, prof.min_segment);
-    if (!p.col_us) {
-      p.col_staged = true;
-      p.col_us = U;
-    }
+    if (!p.col_us) fallback(p.m, p.col_us, p.col_staged);
   } else {
     int64_t W = std::min<int64_t>(p.n, (128 + U - 1) / U);
     if (p.m * W * U > prof.lds_max) W = std::min<int64_t>(p.n, (prof.min_segment + U - 1) / U);
@@ -291,12 +319,9 @@ CatPlan plan_catanzaro(int64_t R, int64_t C, int64_t U, const DeviceProfile& pro
   // row pass
   p.row_staged = false;
   p.row_us = pick_slice(p.n, U, prof.lds_max, prof.min_segment);
-  if (!p.row_us) {
-    p.row_staged = true;
-    p.row_us = U;
-  }
+  if (!p.row_us) fallback(p.n, p.row_us, p.row_staged);
   const double col_cost = p.col_staged ? 2.0 : 1.0, row_cost = p.row_staged ? 2.0 : 1.0;
-  p.cost = row_cost + col_cost * (p.npass - 1);
+  p.cost = row_cost + col_cost * (p.npass - 1) + narrow_cost;
   // a staged pass needs at least one footprint of scratch
   const int64_t col_fp = p.m * p.col_W * p.col_us, row_fp = p.n * p.row_us;
   p.ok = true;

@@ -98,10 +98,11 @@ def test_server_alone_and_in_the_mix(torch_cuda):
 def test_server_refuses_requests_it_cannot_admit(torch_cuda):
     import eithot_amd as E
     torch = torch_cuda
-    # rows of 100003 units exceed LDS and need scratch that neither alpha nor a
-    # 1 KiB pool provides: the request is refused at submit, nothing runs
-    t = torch.zeros(2 * 100003 * 4, dtype=torch.uint8, device="cuda:0")
+    # rows of 200003 8-byte units exceed LDS even as 1-byte slices, so the row
+    # pass is staged through 200003 bytes of scratch, more than a 1 KiB pool
+    # holds: the request is refused at submit, nothing runs
+    t = torch.zeros(3 * 200003 * 8, dtype=torch.uint8, device="cuda:0")
     srv = E.Server(scratch_pool_bytes=1024, max_inflight=2, nstreams=1)
     with pytest.raises(E.EithotError):
-        srv.submit(t.data_ptr(), [100003, 2], [2, 1], 4)
+        srv.submit(t.data_ptr(), [3, 200003], [2, 1], 8)
     srv.close()

@@ -44,6 +44,18 @@ def test_staged_passes(torch_cuda):
         assert peak <= 1.0 * got.nbytes
 
 
+def test_narrow_slice_row_passes(torch_cuda):
+    # rows longer than LDS with a 5 % budget: each unit is moved in narrow
+    # slices (1 or 2 bytes of an 8-byte unit) through LDS, or through scratch
+    # when even 1-byte slices of a row exceed LDS ([3, 200003])
+    for dims, perm, elem in (([9, 271, 362], [3, 2, 1], 8), ([3, 100003], [2, 1], 8),
+                             ([100003, 3], [2, 1], 8), ([2, 50021, 2], [2, 1, 3], 8),
+                             ([3, 200003], [2, 1], 8)):
+        got, want, peak = run_case(torch_cuda, dims, perm, elem, seed=6, alpha=0.05)
+        assert_bitexact(got, want, f"{dims} {perm}")
+        assert peak <= 0.05 * got.nbytes
+
+
 def test_random_high_order(torch_cuda):
     rng = np.random.default_rng(123)
     for _ in range(60):

@@ -99,6 +99,29 @@ def test_staged_catanzaro_plans_execute_correctly():
     assert {(3, 5, 1), (3, 6, 1)} & seen and {(2, 3, 1), (2, 4, 1)} & seen, seen
 
 
+NARROW_CASES = [([9, 271, 362], [3, 2, 1], 8, 0.05, 0), ([3, 100003], [2, 1], 8, 0.05, 0),
+                ([100003, 3], [2, 1], 8, 0.05, 0), ([5, 60013], [2, 1], 4, 0.1, 0),
+                ([2, 50021, 2], [2, 1, 3], 8, 0.05, 0), ([7, 4001], [2, 1], 8, 0.05, 4096),
+                ([3, 200003], [2, 1], 8, 0.05, 0)]
+
+
+@pytest.mark.parametrize("dims,perm1,elem,alpha,lds", NARROW_CASES)
+def test_narrow_slice_plans_execute_correctly(dims, perm1, elem, alpha, lds):
+    # rows too long for LDS at an aligned slice and for the scratch budget as
+    # whole units: the row pass moves narrow slices of each unit through LDS
+    ops, scratch = E.plan_describe(dims, perm1, elem, 1, alpha, lds)
+    narrow = [o for o in ops if o["kind"] == 3 and o["us"] < o["U"]]
+    assert narrow, ops
+    # staged only when 1-byte slices of a row exceed LDS
+    assert all(bool(o["staged"]) == (o["C"] > (lds or 160 * 1024)) for o in narrow), ops
+    vol = int(np.prod(dims))
+    assert scratch <= alpha * vol * elem
+    a = O.fill(vol, elem, 8, 1)
+    b = a.copy()
+    simulate(b, ops)
+    assert np.array_equal(b, O.permute(a, dims, [p - 1 for p in perm1], elem))
+
+
 def test_partition_plan_executes_correctly():
     for dims, perm1, nsub in (([152, 150], [2, 1], 4), ([6, 400, 3], [3, 2, 1], 8),
                               ([96, 40, 5], [2, 3, 1], 4)):
