@@ -300,6 +300,24 @@ def run_serve(count=96, rates=(100, 300, 1000, 3000)):
             "max_inflight": 16, "streams": 4, "sweep": results}
 
 
+def run_copy(nbytes=1 << 30, reps=5):
+    """Plain device-to-device copy of a known size: the calibration kernel for
+    the HBM byte counters (each launch reads and writes nbytes)."""
+    import torch
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty_like(src)
+    src.fill_(7)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    return {"workload": f"device copy of {nbytes} bytes (calibrates FETCH_SIZE / WRITE_SIZE)",
+            "bytes_per_copy": nbytes, "copies": reps, "ms": round(t * 1e3, 3),
+            "GBps_read_plus_write": round(2 * nbytes / t / 1e9, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -308,7 +326,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="c2",
-                    choices=["c2", "c1", "c3", "batch", "serve"],
+                    choices=["c2", "c1", "c3", "batch", "serve", "copy"],
                     help="c2 (default) prints the benchmark line; the others print their own "
                          "JSON report")
     args = ap.parse_args()
@@ -328,6 +346,9 @@ def main():
         return
     if args.workload == "batch":
         print(json.dumps(run_batch()))
+        return
+    if args.workload == "copy":
+        print(json.dumps(run_copy()))
         return
     if args.workload == "serve":
         print(json.dumps(run_serve()))

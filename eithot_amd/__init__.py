@@ -22,7 +22,7 @@ from ._lib import EithotError, check, i32, i64, lib
 
 __all__ = ["transpose", "transpose_tensor", "transpose_ptr", "estimate", "estimate_reference",
            "decompose", "plan_describe", "last_peak_extra_bytes", "fill_pattern",
-           "verify_pattern", "Batch", "Server", "EithotError"]
+           "verify_pattern", "set_tuning", "get_tuning", "Batch", "Server", "EithotError"]
 
 
 def _ptr_of(data):
@@ -119,6 +119,19 @@ def plan_describe(dims, perm1, elem_bytes, n_sub=1, alpha=0.05, lds_max=0):
         o = ops[i]
         out.append({f: getattr(o, f) for f, _ in _lib.OpDesc._fields_})
     return out, scratch.value
+
+
+def set_tuning(key, value):
+    """Process-wide planner/launch knob (``eithot_set_tuning``); returns the old value."""
+    old = get_tuning(key)
+    check(lib().eithot_set_tuning(key.encode(), int(value)))
+    return old
+
+
+def get_tuning(key):
+    v = ctypes.c_int64()
+    check(lib().eithot_get_tuning(key.encode(), ctypes.byref(v)))
+    return v.value
 
 
 def fill_pattern(t, n, elem_bytes, seed, mode, first=0, stream=None):

@@ -27,6 +27,7 @@ EXPORTS = [
     "eithot_transpose", "eithot_transpose_async", "eithot_inplace_transpose",
     "eithot_last_peak_extra_bytes", "eithot_last_error", "eithot_release", "eithot_estimate",
     "eithot_estimate_reference", "eithot_decompose", "eithot_plan_describe",
+    "eithot_set_tuning", "eithot_get_tuning",
     "eithot_set_pass_timing", "eithot_last_pass_times", "eithot_fill_pattern",
     "eithot_verify_pattern", "eithot_batch_create", "eithot_batch_add", "eithot_batch_build",
     "eithot_batch_run", "eithot_batch_info", "eithot_batch_destroy", "eithot_server_create",
@@ -87,6 +88,8 @@ def lib():
                                    ctypes.c_int]
     L.eithot_plan_describe.argtypes = [i64p, ip, ctypes.c_int, sz, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_int64, ctypes.POINTER(OpDesc), ctypes.c_int, i64p]
+    L.eithot_set_tuning.argtypes = [ctypes.c_char_p, ctypes.c_int64]
+    L.eithot_get_tuning.argtypes = [ctypes.c_char_p, i64p]
     L.eithot_set_pass_timing.argtypes = [ctypes.c_int]
     L.eithot_last_pass_times.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int]
     L.eithot_fill_pattern.argtypes = [vp, ctypes.c_uint64, sz, ctypes.c_uint64, ctypes.c_int,

@@ -134,13 +134,15 @@ int eithot_batch_build(eithot_batch* b, double alpha, int tune) {
   if (b->built) return fail(EITHOT_ERR_INVALID, "batch already built");
   if (!(alpha > 0)) return fail(EITHOT_ERR_INVALID, "alpha must be > 0");
   const size_t n = b->tensors.size();
+  const DeviceProfile prof = b->dev->profile();
+  b->cfg = b->dev->launch_cfg();
   b->fwd.resize(n);
   b->inv.resize(n);
   for (size_t t = 0; t < n; ++t) {
     const TensorReq& r = b->tensors[t];
     const int order = (int)r.dims.size();
-    int rc = make_plan(r.dims.data(), r.perm.data(), order, r.elem, 1, alpha, b->dev->prof,
-                       REORDER_MIX, b->fwd[t]);
+    int rc = make_plan(r.dims.data(), r.perm.data(), order, r.elem, 1, alpha, prof, prof.reorder,
+                       b->fwd[t]);
     if (rc) return fail(rc, "planning tensor %zu failed (%d)", t, rc);
     b->scratch_bytes = std::max(b->scratch_bytes, b->fwd[t].scratch_bytes);
     if (tune) {
@@ -150,7 +152,7 @@ int eithot_batch_build(eithot_batch* b, double alpha, int tune) {
         od[k] = r.dims[r.perm[k] - 1];
         ip[r.perm[k] - 1] = k + 1;
       }
-      rc = make_plan(od.data(), ip.data(), order, r.elem, 1, alpha, b->dev->prof, REORDER_MIX,
+      rc = make_plan(od.data(), ip.data(), order, r.elem, 1, alpha, prof, prof.reorder,
                      b->inv[t]);
       if (rc) return fail(rc, "planning inverse of tensor %zu failed (%d)", t, rc);
       b->scratch_bytes = std::max(b->scratch_bytes, b->inv[t].scratch_bytes);
@@ -215,6 +217,10 @@ int eithot_batch_build(eithot_batch* b, double alpha, int tune) {
   b->inv.clear();
   int rc = b->build_stages(b->fwd, b->cfg, b->grid_per_cu, b->stages);
   if (rc) return rc;
+  for (const LaunchStage& st : b->stages)
+    for (const auto& so : st.side)
+      if (prepare_op(*so.first) != hipSuccess)
+        return fail(EITHOT_ERR_DEVICE, "uploading a pass table failed");
   // capture the forward launches once; replays reuse the instantiated graph
   if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return fail(EITHOT_ERR_DEVICE, "graph capture failed to start");

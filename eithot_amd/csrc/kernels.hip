@@ -1053,15 +1053,184 @@ __global__ void k_verify(const T* p, VerifyArgs v, uint64_t seed_mixed, int mode
 }  // namespace
 
 int64_t op_lds_bytes(const Op& op, const LaunchCfg& cfg) {
-  if (op.staged || op.kind == 0) return 0;
+  if (op.staged || op.kind == 0 || op.kind == OP_CYCLE) return 0;
   Args<uint64_t> a;
   Prepared p;
   prepare_lds<uint64_t>(op, nullptr, cfg, a, p);
   return p.lds;
 }
 
+namespace {
+
+// ---------------------------------------------------------------------------
+// cycle-following pass (OP_CYCLE): whole cells move along the cycles of the
+// R x C transpose.  One wave per (matrix, segment, chunk of 64 words of the
+// cell); every lane owns one word of the cell at every position, so lanes
+// never share data and no LDS or barrier is needed.  A wave walks its
+// segment p -> src(p) writing cell(p) <- old cell(src(p)), with D loads in
+// flight: each store to a position waits for the load of the cell that
+// follows it, and loads return in order, so the position's own old cell has
+// been read before it is overwritten.  The last write of a segment takes the
+// cell saved in a boundary slot by k_cycle_save (cut cycles) or the cell the
+// wave read from its first position before writing it (whole cycles).
+// ---------------------------------------------------------------------------
+struct CycArgs {
+  char* data;
+  uint64_t base, RCU, U;
+  const CycleSeg* segs;
+  const uint32_t* slots;
+  char* boundary;
+  uint32_t nseg, nslots, R, C, chunk_bytes;
+  uint64_t B, items, chunks;
+  FD32 fR;
+  FD64 fchunks, fnseg;
+};
+
+template <typename Word>
+__global__ __launch_bounds__(256) void k_cycle_save(CycArgs a) {
+  constexpr uint64_t w = sizeof(Word);
+  const uint64_t wpc = a.U / w;
+  const uint64_t total = a.B * a.nslots * wpc;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
+    const uint64_t cell = t / wpc;  // boundary cell = b * nslots + slot
+    const uint64_t x = t - cell * wpc;
+    const uint64_t b = cell / a.nslots;
+    const uint64_t s = cell - b * a.nslots;
+    *reinterpret_cast<Word*>(a.boundary + cell * a.U + x * w) = *reinterpret_cast<const Word*>(
+        a.data + a.base + b * a.RCU + (uint64_t)a.slots[s] * a.U + x * w);
+  }
+}
+
+template <typename Word, int D>
+__global__ __launch_bounds__(256) void k_cycle(CycArgs a) {
+  constexpr uint64_t w = sizeof(Word);
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t item = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (item >= a.items) return;
+  const uint64_t rest = a.fchunks.div(item);
+  const uint64_t chunk = item - rest * a.chunks;
+  const uint64_t b = a.fnseg.div(rest);
+  const uint64_t s = rest - b * a.nseg;
+  const uint64_t off = chunk * a.chunk_bytes + lane * w;
+  if (off >= a.U) return;
+  char* m = a.data + a.base + b * a.RCU + off;
+  const CycleSeg sg = a.segs[s];
+  uint32_t p = sg.start;
+  const Word temp =
+      sg.src_slot < 0
+          ? *reinterpret_cast<const Word*>(m + (uint64_t)p * a.U)
+          : *reinterpret_cast<const Word*>(a.boundary + (b * a.nslots + (uint64_t)sg.src_slot) * a.U +
+                                           off);
+  for (uint32_t t = 0; t < sg.len; t += D) {
+    Word v[D];
+    uint32_t q[D];
+    uint32_t pp = p;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const uint32_t j = a.fR.div(pp);
+      pp = (pp - j * a.R) * a.C + j;  // src(pp): (i, j) -> i*C + j
+      q[k] = pp;
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+      if (t + k < sg.len)
+        v[k] = t + k + 1 == sg.len ? temp : *reinterpret_cast<const Word*>(m + (uint64_t)q[k] * a.U);
+    uint32_t dst = p;
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+      if (t + k < sg.len) {
+        *reinterpret_cast<Word*>(m + (uint64_t)dst * a.U) = v[k];
+        dst = q[k];
+      }
+    p = q[D - 1];
+  }
+}
+
+const void* cycle_table_device(const CycleTable& t, hipError_t* err) {
+  *err = hipSuccess;
+  int dev = 0;
+  if ((*err = hipGetDevice(&dev)) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(t.dev_mu);
+  auto it = t.dev.find(dev);
+  if (it != t.dev.end()) return it->second.get();
+  const size_t sb = t.segs.size() * sizeof(CycleSeg), lb = t.slots.size() * sizeof(uint32_t);
+  void* p = nullptr;
+  if ((*err = hipMalloc(&p, std::max<size_t>(sb + lb, 16))) != hipSuccess) return nullptr;
+  std::shared_ptr<void> own(p, [](void* q) { (void)hipFree(q); });
+  if (sb && (*err = hipMemcpy(p, t.segs.data(), sb, hipMemcpyHostToDevice)) != hipSuccess)
+    return nullptr;
+  if (lb && (*err = hipMemcpy((char*)p + sb, t.slots.data(), lb, hipMemcpyHostToDevice)) !=
+                hipSuccess)
+    return nullptr;
+  t.dev[dev] = own;
+  return p;
+}
+
+template <typename Word>
+hipError_t launch_cycle_w(CycArgs a, hipStream_t stream) {
+  if (a.nslots) {
+    const uint64_t words = a.B * a.nslots * (a.U / sizeof(Word));
+    const uint64_t blocks = std::min<uint64_t>((words + 255) / 256, 256ull * 64);
+    hipLaunchKernelGGL((k_cycle_save<Word>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
+  }
+  const uint64_t blocks = (a.items + 3) / 4;
+  hipLaunchKernelGGL((k_cycle<Word, 8>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_cycle(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
+                        hipStream_t stream) {
+  if (!op.cyc || (uint64_t)op.R * (uint64_t)op.C >= (1ull << 32)) return hipErrorInvalidValue;
+  const CycleTable& t = *op.cyc;
+  if (t.segs.empty()) return hipSuccess;  // only fixed points
+  const uint64_t need = (uint64_t)op.B * t.slots.size() * (uint64_t)op.U;
+  if (need && (!scratch || (uint64_t)scratch_bytes < need)) return hipErrorInvalidValue;
+  hipError_t e;
+  const char* d = (const char*)cycle_table_device(t, &e);
+  if (!d) return e;
+  const int64_t w = word_bytes(op, (uintptr_t)data);
+  CycArgs a;
+  a.data = data;
+  a.base = op.base;
+  a.RCU = (uint64_t)op.R * op.C * op.U;
+  a.U = op.U;
+  a.segs = reinterpret_cast<const CycleSeg*>(d);
+  a.slots = reinterpret_cast<const uint32_t*>(d + t.segs.size() * sizeof(CycleSeg));
+  a.boundary = scratch;
+  a.nseg = (uint32_t)t.segs.size();
+  a.nslots = (uint32_t)t.slots.size();
+  a.R = (uint32_t)op.R;
+  a.C = (uint32_t)op.C;
+  a.chunk_bytes = (uint32_t)(64 * w);
+  a.B = op.B;
+  a.chunks = (op.U + a.chunk_bytes - 1) / a.chunk_bytes;
+  a.items = a.B * a.nseg * a.chunks;
+  a.fR.init(a.R);
+  a.fchunks.init(a.chunks);
+  a.fnseg.init(a.nseg);
+  if ((a.items + 3) / 4 >= (1ull << 31)) return hipErrorInvalidValue;
+  switch (w) {
+    case 16: return launch_cycle_w<W16>(a, stream);
+    case 8: return launch_cycle_w<uint64_t>(a, stream);
+    case 4: return launch_cycle_w<uint32_t>(a, stream);
+    case 2: return launch_cycle_w<uint16_t>(a, stream);
+    default: return launch_cycle_w<uint8_t>(a, stream);
+  }
+}
+
+}  // namespace
+
+hipError_t prepare_op(const Op& op) {
+  if (op.kind != OP_CYCLE || !op.cyc || op.cyc->segs.empty()) return hipSuccess;
+  hipError_t e;
+  cycle_table_device(*op.cyc, &e);
+  return e;
+}
+
 hipError_t launch_op(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
                      const LaunchCfg& cfg, hipStream_t stream) {
+  if (op.kind == OP_CYCLE) return launch_cycle(op, data, scratch, scratch_bytes, stream);
   if (op.kind == 0 || op.us <= 0 || op.U % op.us) return hipErrorInvalidValue;
   if (needs64(op)) return run_op<uint64_t>(op, data, scratch, scratch_bytes, cfg, stream);
   return run_op<uint32_t>(op, data, scratch, scratch_bytes, cfg, stream);
@@ -1089,7 +1258,7 @@ int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, cons
   bool big = false;
   for (size_t i = 0; i < ops.size(); ++i) {
     const Op& op = *ops[i].first;
-    if (op.staged || op.kind == 0 || needs64(op)) {
+    if (op.staged || op.kind == 0 || op.kind == OP_CYCLE || needs64(op)) {
       if (rejected) rejected->push_back(i);
       continue;
     }

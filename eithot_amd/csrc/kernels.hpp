@@ -23,6 +23,11 @@ struct LaunchCfg {
 hipError_t launch_op(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
                      const LaunchCfg& cfg, hipStream_t stream);
 
+// Uploads what an op needs on the current device before its first launch
+// (the cycle table of OP_CYCLE; nothing for other kinds).  Call before
+// capturing the op into a graph.
+hipError_t prepare_op(const Op& op);
+
 // Bytes of LDS the op's LDS kernel uses per workgroup with cfg (0 if staged).
 int64_t op_lds_bytes(const Op& op, const LaunchCfg& cfg);
 

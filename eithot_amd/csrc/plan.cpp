@@ -4,7 +4,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <tuple>
 
@@ -192,6 +194,7 @@ int64_t Op::footprint_units() const {
     case OP_TILE: return R * C;
     case OP_COL: return R * W;
     case OP_ROW: return C;
+    case OP_CYCLE: return 1;  // scratch: one saved cell per boundary slot
   }
   return 0;
 }
@@ -202,8 +205,49 @@ int64_t Op::num_footprints() const {
     case OP_TILE: return B * slices;
     case OP_COL: return B * ((C + W - 1) / W) * slices;
     case OP_ROW: return B * R * slices;
+    case OP_CYCLE: return cyc ? B * (int64_t)cyc->slots.size() : 0;
   }
   return 0;
+}
+
+std::shared_ptr<CycleTable> build_cycle_table(int64_t R, int64_t C, int64_t lmax) {
+  auto t = std::make_shared<CycleTable>();
+  t->R = R;
+  t->C = C;
+  t->lmax = lmax;
+  const uint64_t N = (uint64_t)R * (uint64_t)C;
+  if (N >= (1ull << 32) || lmax < 1) return t;
+  auto src = [&](uint64_t p) { return (p % (uint64_t)R) * (uint64_t)C + p / (uint64_t)R; };
+  std::vector<uint8_t> seen(N, 0);
+  for (uint64_t p0 = 0; p0 < N; ++p0) {
+    if (seen[p0]) continue;
+    uint64_t k = 0, q = p0;
+    do {
+      seen[q] = 1;
+      q = src(q);
+      ++k;
+    } while (q != p0);
+    if (k == 1) continue;
+    if ((int64_t)k <= lmax) {
+      t->segs.push_back({(uint32_t)p0, (uint32_t)k, -1, 0});
+      continue;
+    }
+    // cut into n nearly equal segments; segment i starts at q_i and takes
+    // its last cell from the slot of segment i+1 (cyclically)
+    const uint64_t n = (k + (uint64_t)lmax - 1) / (uint64_t)lmax;
+    const uint32_t first_slot = (uint32_t)t->slots.size();
+    const size_t first_seg = t->segs.size();
+    q = p0;
+    for (uint64_t i = 0; i < n; ++i) {
+      const uint64_t len = k / n + (i < k % n ? 1 : 0);
+      t->slots.push_back((uint32_t)q);
+      t->segs.push_back({(uint32_t)q, (uint32_t)len,
+                         (int32_t)(first_slot + (uint32_t)((i + 1) % n)), 0});
+      for (uint64_t s = 0; s < len; ++s) q = src(q);
+    }
+    (void)first_seg;
+  }
+  return t;
 }
 
 namespace {
@@ -233,19 +277,41 @@ int64_t pick_slice(int64_t units, int64_t U, int64_t budget, int64_t min_seg) {
   return best;
 }
 
-// Tile footprint (R rows of C slices, padded rows) that fits `budget`.
-int64_t pick_tile_slice(int64_t R, int64_t C, int64_t U, int64_t budget, int64_t min_seg) {
-  auto fits = [&](int64_t us) { return R * tile_row_stride_bound(C * us, us) <= budget; };
-  if (fits(U)) return U;
+// Tile footprint (R rows of C slices, padded rows): whole units if they fit
+// the footprint target; else the largest aligned slice of at least slice_min
+// bytes that does; else whole units if they fit LDS; else the largest
+// aligned slice of at least min_segment bytes that fits LDS.  0 if none.
+int64_t pick_tile_slice(int64_t R, int64_t C, int64_t U, const DeviceProfile& prof) {
+  auto fp = [&](int64_t us) { return R * tile_row_stride_bound(C * us, us); };
   const int64_t align = pow2_factor(U, 16);
-  int64_t best = 0;
-  for (int64_t q = 1; q * q <= U; ++q) {
-    if (U % q) continue;
-    for (int64_t us : {q, U / q})
-      if (us % align == 0 && us >= min_seg && us > best && fits(us)) best = us;
-  }
-  return best;
+  auto largest = [&](int64_t cap, int64_t min_seg) {
+    int64_t best = 0;
+    for (int64_t q = 1; q * q <= U; ++q) {
+      if (U % q) continue;
+      for (int64_t us : {q, U / q})
+        if (us % align == 0 && us >= min_seg && us > best && fp(us) <= cap) best = us;
+    }
+    return best;
+  };
+  const int64_t target = prof.tile_fp_target > 0 ? std::min(prof.tile_fp_target, prof.lds_max)
+                                                 : prof.lds_max;
+  if (fp(U) <= target) return U;
+  if (int64_t us = largest(target, std::max(prof.slice_min, prof.min_segment))) return us;
+  if (fp(U) <= prof.lds_max) return U;
+  return largest(prof.lds_max, prof.min_segment);
 }
+
+// Slice of a staged TILE: the largest aligned slice (at least min(U, 256)
+// bytes) whose footprint fits both the scratch budget and the staging chunk.
+int64_t pick_staged_tile_slice(int64_t R, int64_t C, int64_t U, int64_t budget,
+                               const DeviceProfile& prof) {
+  if (prof.stage_chunk > 0) {
+    const int64_t min_seg = std::min<int64_t>(U, std::max(prof.slice_min, prof.min_segment));
+    if (int64_t us = pick_slice(R * C, U, std::min(budget, prof.stage_chunk), min_seg)) return us;
+  }
+  return pick_slice(R * C, U, budget, std::min<int64_t>(U, 256));
+}
+
 
 // Largest divisor of U whose `units` slices fit `cap` bytes, whatever its
 // alignment (narrow slices cost word width, but always exist).  0 if none.
@@ -349,16 +415,23 @@ enum How {
   HOW_CAT = 2,
   HOW_SPLIT_R = 3,
   HOW_SPLIT_C = 4,
-  HOW_TILE_STAGED = 5
+  HOW_TILE_STAGED = 5,
+  HOW_CYCLE = 6
 };
 
 // Relative time of one pass over the tensor, by pass class (contiguous LDS
-// tile = 1).  Sliced tiles move short segments; staged passes cross HBM
-// twice; Catanzaro passes evaluate heavier index maps.
+// tile = 1), from MI355X measurements: sliced tiles move short scattered
+// segments (about 2.3x a contiguous tile), staged passes cross HBM twice;
+// Catanzaro passes evaluate heavier index maps.
 constexpr double kCostTile = 1.0;
-constexpr double kCostTileSliced = 1.3;
-constexpr double kCostTileStaged = 2.05;
+constexpr double kCostTileSliced = 2.2;
+constexpr double kCostTileStaged = 2.1;
 constexpr double kCostSplit = 0.001;
+
+// Cycle-following moves whole cells (of at least cycle_min_cell bytes), one
+// read and one write each; smaller cells waste HBM bursts.
+constexpr uint64_t kCycleMaxCells = 1ull << 26;
+inline double cycle_cost(int64_t U) { return 1.05 + 256.0 / (double)U; }
 
 struct Choice {
   int how = HOW_NONE;
@@ -383,7 +456,7 @@ struct Lowerer {
     auto key = std::make_tuple(R, C, U, depth);
     auto it = memo.find(key);
     if (it != memo.end()) return it->second;
-    if (int64_t us = pick_tile_slice(R, C, U, prof.lds_max, prof.min_segment)) {
+    if (int64_t us = pick_tile_slice(R, C, U, prof)) {
       ch.how = HOW_TILE;
       ch.cost = us == U ? kCostTile : kCostTileSliced;
       if (us == U) {
@@ -396,9 +469,16 @@ struct Lowerer {
       ch.how = HOW_CAT;
       ch.cost = cp.cost;
     }
+    // whole cells along the cycles of the transpose, no LDS
+    if (prof.cycle_min_cell > 0 && U >= prof.cycle_min_cell &&
+        (uint64_t)R * (uint64_t)C <= kCycleMaxCells &&
+        cycle_cost(U) < ch.cost) {
+      ch.how = HOW_CYCLE;
+      ch.cost = cycle_cost(U);
+    }
     // whole matrices (or slices) through scratch: copy out, tiled transpose back
     if (ch.cost > kCostTileStaged) {
-      const int64_t us = pick_slice(R * C, U, scratch_budget, std::min<int64_t>(U, 256));
+      const int64_t us = pick_staged_tile_slice(R, C, U, scratch_budget, prof);
       if (us) {
         ch.how = HOW_TILE_STAGED;
         ch.cost = kCostTileStaged;
@@ -434,6 +514,25 @@ struct Lowerer {
     return ch;
   }
 
+  // Segment length: short enough for about 32 Ki waves (one per segment and
+  // 1 KiB of cell), long enough that the saved boundary cells fit the budget.
+  std::shared_ptr<const CycleTable> cycle_table_for(int64_t B, int64_t R, int64_t C, int64_t U) {
+    const int64_t chunk = 64 * pow2_factor(U, 16);
+    const int64_t chunks = (U + chunk - 1) / chunk;
+    int64_t lmax = 256;
+    std::shared_ptr<CycleTable> t = build_cycle_table(R, C, lmax);
+    while (lmax > 16 && B * (int64_t)t->segs.size() * chunks < 32768) {
+      lmax /= 2;
+      t = build_cycle_table(R, C, lmax);
+    }
+    const int64_t N = R * C;
+    while (!t->slots.empty() && B * (int64_t)t->slots.size() * U > scratch_budget) {
+      lmax = lmax >= N ? N : lmax * 2;
+      t = build_cycle_table(R, C, lmax);
+    }
+    return t;
+  }
+
   void emit(int64_t B, int64_t R, int64_t C, int64_t U, uint64_t base, int stage, int depth,
             std::vector<Op>& ops) {
     if (R <= 1 || C <= 1 || B <= 0) return;
@@ -448,7 +547,7 @@ struct Lowerer {
         op.R = R;
         op.C = C;
         op.U = U;
-        op.us = pick_tile_slice(R, C, U, prof.lds_max, prof.min_segment);
+        op.us = pick_tile_slice(R, C, U, prof);
         ops.push_back(op);
         return;
       }
@@ -462,7 +561,21 @@ struct Lowerer {
         op.R = R;
         op.C = C;
         op.U = U;
-        op.us = pick_slice(R * C, U, scratch_budget, std::min<int64_t>(U, 256));
+        op.us = pick_staged_tile_slice(R, C, U, scratch_budget, prof);
+        ops.push_back(op);
+        return;
+      }
+      case HOW_CYCLE: {
+        Op op;
+        op.kind = OP_CYCLE;
+        op.stage = stage;
+        op.base = base;
+        op.B = B;
+        op.R = R;
+        op.C = C;
+        op.U = U;
+        op.us = U;
+        op.cyc = cycle_table_for(B, R, C, U);
         ops.push_back(op);
         return;
       }
@@ -537,16 +650,76 @@ struct Lowerer {
 
 }  // namespace
 
+// ---- tuning knobs --------------------------------------------------------------
+namespace {
+std::mutex g_tune_mu;
+Tuning g_tune;
+}  // namespace
+
+Tuning tuning_get() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return g_tune;
+}
+
+static int64_t* tuning_field(Tuning& t, const char* key) {
+  if (!key) return nullptr;
+  if (!strcmp(key, "lds_target")) return &t.lds_target;
+  if (!strcmp(key, "tile_fp_target")) return &t.tile_fp_target;
+  if (!strcmp(key, "slice_min")) return &t.slice_min;
+  if (!strcmp(key, "stage_chunk")) return &t.stage_chunk;
+  if (!strcmp(key, "cycle_min_cell")) return &t.cycle_min_cell;
+  if (!strcmp(key, "threads")) return &t.threads;
+  if (!strcmp(key, "threads_big")) return &t.threads_big;
+  if (!strcmp(key, "xcd_remap")) return &t.xcd_remap;
+  if (!strcmp(key, "reorder")) return &t.reorder;
+  return nullptr;
+}
+
+int tuning_set(const char* key, int64_t value) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  Tuning t = g_tune;
+  int64_t* f = tuning_field(t, key);
+  if (!f || value < 0) return EITHOT_ERR_INVALID;
+  const bool threads = f == &t.threads || f == &t.threads_big;
+  if (threads && (value < 64 || value > 1024 || value % 64)) return EITHOT_ERR_INVALID;
+  if (f == &t.reorder && value > 2) return EITHOT_ERR_INVALID;
+  if (f == &t.lds_target && value < 1024) return EITHOT_ERR_INVALID;
+  *f = value;
+  t.epoch = g_tune.epoch + 1;
+  g_tune = t;
+  return EITHOT_OK;
+}
+
+int tuning_value(const char* key, int64_t* value) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  Tuning t = g_tune;
+  int64_t* f = tuning_field(t, key);
+  if (!f) return EITHOT_ERR_INVALID;
+  if (value) *value = *f;
+  return EITHOT_OK;
+}
+
+void tuning_apply(const Tuning& t, DeviceProfile& prof) {
+  prof.lds_target = std::min<int64_t>(t.lds_target, prof.lds_max);
+  prof.tile_fp_target = t.tile_fp_target;
+  prof.slice_min = t.slice_min;
+  prof.stage_chunk = t.stage_chunk;
+  prof.cycle_min_cell = t.cycle_min_cell;
+  prof.reorder = (int)t.reorder;
+}
+
 void lower_transpose2d(int64_t B, int64_t R, int64_t C, int64_t U, uint64_t base, int stage,
                        const DeviceProfile& prof, std::vector<Op>& ops) {
   Lowerer L(prof, INT64_MAX / 4);
   L.emit(B, R, C, U, base, stage, 0, ops);
 }
 
-static int64_t staged_scratch(const Op& op, int64_t budget) {
+static int64_t staged_scratch(const Op& op, int64_t budget, const DeviceProfile& prof) {
+  if (op.kind == OP_CYCLE) return op.cyc ? op.B * (int64_t)op.cyc->slots.size() * op.U : 0;
   if (!op.staged) return 0;
   const int64_t fp = op.footprint_bytes();
   const int64_t nfp = op.num_footprints();
+  if (prof.stage_chunk > 0) budget = std::min(budget, std::max(prof.stage_chunk, fp));
   int64_t g = std::max<int64_t>(1, budget / std::max<int64_t>(fp, 1));
   g = std::min(g, nfp);
   return g * fp;
@@ -619,14 +792,15 @@ int make_plan(const int64_t* dims_in, const int* perm1, int rank, size_t elem, i
   }
   for (const Op& op : plan.ops) {
     if (op.kind == 0) return EITHOT_ERR_ALPHA;
-    plan.scratch_bytes = std::max(plan.scratch_bytes, staged_scratch(op, budget));
+    plan.scratch_bytes = std::max(plan.scratch_bytes, staged_scratch(op, budget, prof));
+    if (op.cyc) plan.table_bytes += op.cyc->bytes();
   }
   return EITHOT_OK;
 }
 
 std::string Plan::describe() const {
   std::ostringstream os;
-  static const char* kn[] = {"?", "tile", "col", "row"};
+  static const char* kn[] = {"?", "tile", "col", "row", "cycle"};
   static const char* mn[] = {"-", "rot", "rotinv", "colsh", "colshinv", "rowg", "rowd"};
   os << "stages:";
   for (const Stage& s : stages) {
@@ -747,7 +921,7 @@ struct RefModel {
   // Nd_transpose.cpp (estimator twin):156-249, max over rounds
   int64_t series(const int64_t* dims, const int* perm1, int rank) const {
     std::vector<Stage> st;
-    decompose(dims, perm1, rank, REORDER_MIX, st);
+    decompose(dims, perm1, rank, (int)tuning_get().reorder, st);
     int64_t m = 0;
     for (const Stage& s : st) m = std::max(m, stage(s));
     return m;
@@ -842,7 +1016,7 @@ int estimate_native(const int64_t* dims, const int* perm1, int rank, size_t elem
                     const DeviceProfile& prof, int* n_sub, double* alpha_out,
                     size_t* extra_bytes) {
   Plan plan;
-  int rc = make_plan(dims, perm1, rank, elem, 1, alpha, prof, REORDER_MIX, plan);
+  int rc = make_plan(dims, perm1, rank, elem, 1, alpha, prof, prof.reorder, plan);
   if (rc != EITHOT_OK) return rc;
   if (n_sub) *n_sub = plan.n_sub;
   if (extra_bytes) *extra_bytes = (size_t)plan.scratch_bytes;

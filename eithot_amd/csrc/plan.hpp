@@ -9,6 +9,9 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -32,7 +35,7 @@ bool decompose(const int64_t* dims, const int* perm1, int rank, int reorder,
                std::vector<Stage>& out);
 
 // ---- device passes ---------------------------------------------------------
-enum OpKind : int { OP_TILE = 1, OP_COL = 2, OP_ROW = 3 };
+enum OpKind : int { OP_TILE = 1, OP_COL = 2, OP_ROW = 3, OP_CYCLE = 4 };
 
 // Index maps of the passes, all in gather form (dest <- source).
 enum MapKind : int {
@@ -44,6 +47,31 @@ enum MapKind : int {
   MAP_ROWG = 5,        // C2R row shuffle (gather form)
   MAP_ROWD = 6,        // R2C row shuffle (gather form)
 };
+
+// Cycle structure of the R x C transpose of cells (OP_CYCLE).  In gather
+// form output position p = j*R + i takes input position src(p) = i*C + j;
+// following p -> src(p) walks a cycle.  Cycles longer than `lmax` are cut
+// into segments; the first position of every cut segment gets a boundary
+// slot whose old cell is saved before the main pass, because the segment
+// before it in the cycle reads it last.
+struct CycleSeg {
+  uint32_t start;    // first position written
+  uint32_t len;      // positions written: start, src(start), ...
+  int32_t src_slot;  // slot holding the old cell the last write takes; -1: the
+                     // segment is a whole cycle and takes old(start)
+  uint32_t pad;
+};
+struct CycleTable {
+  int64_t R = 0, C = 0, lmax = 0;
+  std::vector<CycleSeg> segs;
+  std::vector<uint32_t> slots;  // slot -> position whose old cell it holds
+  int64_t bytes() const { return (int64_t)(segs.size() * sizeof(CycleSeg) + slots.size() * 4); }
+  // device copies (segs then slots), one per device id, made by the launcher
+  mutable std::mutex dev_mu;
+  mutable std::map<int, std::shared_ptr<void>> dev;
+};
+// Builds the table (fixed points are skipped).  R*C must be below 2^32.
+std::shared_ptr<CycleTable> build_cycle_table(int64_t R, int64_t C, int64_t lmax);
 
 struct Op {
   int kind = 0;
@@ -58,6 +86,8 @@ struct Op {
   // Catanzaro constants of the m x n grid (m = R, n = C of this op's grid)
   int64_t m = 1, n = 1, c = 1, a = 1, b = 1, k = 0, binv = 0;
 
+  std::shared_ptr<const CycleTable> cyc;  // OP_CYCLE: segments shared by all B matrices
+
   int64_t footprint_units() const;   // units staged per footprint
   int64_t footprint_bytes() const { return footprint_units() * us; }
   int64_t num_footprints() const;
@@ -69,7 +99,34 @@ struct DeviceProfile {
   int64_t lds_target = 64 * 1024; // per-workgroup LDS the launcher packs footprints to
   int cus = 256;
   int64_t min_segment = 64;       // smallest contiguous run worth a footprint slice
+  // Tile footprints above this are sliced (slices of at least slice_min
+  // bytes) so that several workgroups share a CU and overlap their load and
+  // store phases; 0 keeps whole units whenever they fit lds_max.
+  int64_t tile_fp_target = 64 * 1024;
+  int64_t slice_min = 128;
+  int64_t stage_chunk = 0;        // > 0: bytes of scratch one staged group may fill
+  int64_t cycle_min_cell = 512;   // smallest cell moved by cycle following (0: off)
+  int reorder = 0;                // decomposer order (Reorder)
 };
+
+// Process-wide tuning knobs (eithot_set_tuning); applied over the hardware
+// profile of every device and over the host-only planning entry points.
+struct Tuning {
+  int64_t lds_target = 64 * 1024;
+  int64_t tile_fp_target = 64 * 1024;
+  int64_t slice_min = 128;
+  int64_t stage_chunk = 0;
+  int64_t cycle_min_cell = 512;
+  int64_t threads = 256;
+  int64_t threads_big = 512;
+  int64_t xcd_remap = 1;
+  int64_t reorder = 0;
+  uint64_t epoch = 0;  // bumped on every change (plan caches key on it)
+};
+Tuning tuning_get();
+int tuning_set(const char* key, int64_t value);  // 0 or EITHOT_ERR_INVALID
+int tuning_value(const char* key, int64_t* value);
+void tuning_apply(const Tuning& t, DeviceProfile& prof);
 
 struct Plan {
   std::vector<Stage> stages;
@@ -77,6 +134,7 @@ struct Plan {
   int n_sub = 1;              // sub-tensors actually used
   int partition_dim = -1;     // 0-based input dim split into n_sub blocks
   int64_t scratch_bytes = 0;  // device scratch the plan needs (0 if all LDS)
+  int64_t table_bytes = 0;    // device cycle tables of its OP_CYCLE passes
   uint64_t tensor_bytes = 0;
   std::string describe() const;
 };

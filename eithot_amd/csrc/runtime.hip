@@ -76,9 +76,27 @@ Device* device_ctx(int* err) {
   return devs[id].get();
 }
 
+DeviceProfile Device::profile() const {
+  DeviceProfile p = prof;
+  tuning_apply(tuning_get(), p);
+  return p;
+}
+
+LaunchCfg Device::launch_cfg() const {
+  const Tuning t = tuning_get();
+  LaunchCfg c = cfg;
+  c.lds_target = std::min<int64_t>(t.lds_target, prof.lds_max);
+  c.threads = (int)t.threads;
+  c.threads_big = (int)t.threads_big;
+  c.xcd_remap = (int)t.xcd_remap;
+  return c;
+}
+
 const Plan* Device::plan_for(const int64_t* dims, const int* perm1, int rank, size_t elem,
                              int n_sub, double alpha, int* rc) {
+  const DeviceProfile pr = profile();
   std::ostringstream k;
+  k << tuning_get().epoch << "#";
   for (int i = 0; i < rank; ++i) k << dims[i] << ",";
   k << "|";
   for (int i = 0; i < rank; ++i) k << perm1[i] << ",";
@@ -91,7 +109,7 @@ const Plan* Device::plan_for(const int64_t* dims, const int* perm1, int rank, si
     return it->second.get();
   }
   auto p = std::make_unique<Plan>();
-  *rc = make_plan(dims, perm1, rank, elem, n_sub, alpha, prof, REORDER_MIX, *p);
+  *rc = make_plan(dims, perm1, rank, elem, n_sub, alpha, pr, pr.reorder, *p);
   if (*rc != EITHOT_OK) return nullptr;
   if (plans.size() > 4096) plans.clear();
   const Plan* raw = p.get();
@@ -140,8 +158,9 @@ int run_plan(Device* d, const Plan& plan, char* data, char* scratch, int64_t scr
     hipEventRecord(ev[0], stream);
   }
   int rc = EITHOT_OK;
+  const LaunchCfg cfg = d->launch_cfg();
   for (size_t i = 0; i < plan.ops.size(); ++i) {
-    hipError_t e = launch_op(plan.ops[i], data, scratch, scratch_bytes, d->cfg, stream);
+    hipError_t e = launch_op(plan.ops[i], data, scratch, scratch_bytes, cfg, stream);
     if (e != hipSuccess) {
       rc = fail(EITHOT_ERR_DEVICE, "pass %zu launch failed: %s", i, hipGetErrorString(e));
       break;
@@ -210,7 +229,7 @@ static int transpose_impl(void* dev, const int64_t* dims, const int* perm, int o
     rc = d->release_arena_after(stream);
     if (rc) return rc;
   }
-  g_peak = (size_t)plan->scratch_bytes;
+  g_peak = (size_t)(plan->scratch_bytes + plan->table_bytes);
   if (sync) {
     hipError_t e = hipStreamSynchronize(stream);
     if (e == hipSuccess) e = hipGetLastError();
@@ -270,6 +289,7 @@ void eithot_release(void) {
 int eithot_estimate(const int64_t* dims, const int* perm, int order, size_t elem_bytes,
                     double alpha, int* n_sub_out, double* alpha_out, size_t* extra_bytes_out) {
   DeviceProfile prof;
+  tuning_apply(tuning_get(), prof);
   int rc = estimate_native(dims, perm, order, elem_bytes, alpha, prof, n_sub_out, alpha_out,
                            extra_bytes_out);
   if (rc) return fail(rc, "estimate failed (%d)", rc);
@@ -320,12 +340,10 @@ int eithot_plan_describe(const int64_t* dims, const int* perm, int order, size_t
                          int n_sub, double alpha, int64_t lds_max, eithot_op_desc* ops,
                          int max_ops, int64_t* scratch_bytes) {
   DeviceProfile prof;
-  if (lds_max > 0) {
-    prof.lds_max = lds_max;
-    prof.lds_target = std::min<int64_t>(prof.lds_target, lds_max);
-  }
+  if (lds_max > 0) prof.lds_max = lds_max;
+  tuning_apply(tuning_get(), prof);
   Plan plan;
-  int rc = make_plan(dims, perm, order, elem_bytes, n_sub, alpha, prof, REORDER_MIX, plan);
+  int rc = make_plan(dims, perm, order, elem_bytes, n_sub, alpha, prof, prof.reorder, plan);
   if (rc) return fail(rc, "planning failed (%d)", rc);
   if (scratch_bytes) *scratch_bytes = plan.scratch_bytes;
   const int n = (int)plan.ops.size();
@@ -352,6 +370,18 @@ int eithot_plan_describe(const int64_t* dims, const int* perm, int order, size_t
     x.binv = o.binv;
   }
   return n;
+}
+
+int eithot_set_tuning(const char* key, int64_t value) {
+  int rc = tuning_set(key, value);
+  if (rc) return fail(rc, "unknown tuning key or value out of range");
+  return EITHOT_OK;
+}
+
+int eithot_get_tuning(const char* key, int64_t* value) {
+  int rc = tuning_value(key, value);
+  if (rc) return fail(rc, "unknown tuning key");
+  return EITHOT_OK;
 }
 
 void eithot_set_pass_timing(int enable) { g_timing = enable ? 1 : 0; }

@@ -26,6 +26,9 @@ struct Device {
 
   const Plan* plan_for(const int64_t* dims, const int* perm1, int rank, size_t elem, int n_sub,
                        double alpha, int* rc);
+  // hardware profile / launch configuration with the tuning knobs applied
+  DeviceProfile profile() const;
+  LaunchCfg launch_cfg() const;
   int reserve_arena(int64_t bytes, hipStream_t stream);
   int release_arena_after(hipStream_t stream);
 };

@@ -123,6 +123,22 @@ int eithot_plan_describe(const int64_t* dims, const int* perm, int order, size_t
                          int n_sub, double alpha, int64_t lds_max, eithot_op_desc* ops,
                          int max_ops, int64_t* scratch_bytes);
 
+/* ---- tuning -------------------------------------------------------------------
+ * No counterpart in the reference, whose equivalents are compile-time
+ * constants (REORDER_TYPE, Nd_transpose.cpp:28; tier thresholds in col_op.cu
+ * and row_op.cu).  Process-wide knobs, applied to plans made after the call:
+ *   "reorder"        decomposer order: 0 mix (default), 1 pre, 2 post
+ *   "lds_target"     LDS bytes a workgroup packs footprints up to (65536)
+ *   "tile_fp_target" tile footprints above this are sliced (65536; 0 = off)
+ *   "slice_min"      smallest slice such slicing may choose (128)
+ *   "stage_chunk"    > 0: bytes of scratch one staged group may fill (0)
+ *   "cycle_min_cell" smallest cell (bytes) moved by cycle following (512; 0 = off)
+ *   "threads", "threads_big"  workgroup sizes (256, 512; multiples of 64)
+ *   "xcd_remap"      keep consecutive footprints on one XCD (1)
+ * Returns EITHOT_ERR_INVALID for an unknown key or an out-of-range value. */
+int eithot_set_tuning(const char* key, int64_t value);
+int eithot_get_tuning(const char* key, int64_t* value);
+
 /* ---- measurement helpers ----------------------------------------------------- */
 
 /* When enabled, the next transposes record per-pass GPU time (HIP events on

@@ -1,0 +1,56 @@
+"""The cycle-following pass (OP_CYCLE): the planner's cycle tables, and a
+host emulation of the device kernel's step order, against the oracle's
+direct transpose."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(HERE, "_build", "libcycles_host.so")
+
+
+@pytest.fixture(scope="module")
+def ch():
+    src = os.path.join(HERE, "cycles_harness.cpp")
+    deps = [src] + [os.path.join(HERE, "..", "eithot_amd", "csrc", f) for f in ("plan.cpp", "plan.hpp")]
+    if not os.path.exists(SO) or os.path.getmtime(SO) < max(os.path.getmtime(d) for d in deps):
+        os.makedirs(os.path.dirname(SO), exist_ok=True)
+        subprocess.check_call(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-o", SO, src,
+                               deps[1]])
+    L = ctypes.CDLL(SO)
+    L.ch_run.restype = ctypes.c_int64
+    L.ch_run.argtypes = [ctypes.c_int64] * 6 + [ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p,
+                                                ctypes.POINTER(ctypes.c_int64)]
+    L.ch_check.argtypes = [ctypes.c_int64] * 3
+    return L
+
+
+SHAPES = [(2, 3), (3, 2), (8, 36), (36, 8), (48, 320), (64, 100), (7, 11), (16, 16), (1, 9),
+          (1440, 8), (11520, 64)]
+
+
+@pytest.mark.parametrize("R,C", SHAPES)
+@pytest.mark.parametrize("lmax", [1 << 30, 64, 5, 1])
+def test_cycle_tables_cover_every_moved_position_once(ch, R, C, lmax):
+    assert ch.ch_check(R, C, lmax) == 0
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("B,R,C,U,lmax,chunk", [
+    (3, 8, 36, 48, 1 << 30, 16), (2, 36, 8, 40, 7, 16), (1, 48, 320, 32, 64, 16),
+    (2, 13, 17, 24, 3, 8), (1, 64, 100, 16, 9, 16), (4, 5, 7, 64, 1, 64), (1, 120, 64, 16, 100, 16)])
+def test_cycle_emulation_equals_transpose(ch, mode, B, R, C, U, lmax, chunk):
+    vol = B * R * C * U
+    a = O.fill(vol, 1, 17 + mode, 1)
+    want = a.copy()
+    assert O.transpose2d_direct(want, B, R, C, U) == 0
+    b = a.copy()
+    ns = ctypes.c_int64()
+    segs = ch.ch_run(B, R, C, U, lmax, chunk, mode, 5, b.ctypes.data, ctypes.byref(ns))
+    assert segs >= 0
+    assert np.array_equal(b, want), (B, R, C, U, lmax, mode, segs, ns.value)

@@ -48,11 +48,36 @@ def test_narrow_slice_row_passes(torch_cuda):
     # rows longer than LDS with a 5 % budget: each unit is moved in narrow
     # slices (1 or 2 bytes of an 8-byte unit) through LDS, or through scratch
     # when even 1-byte slices of a row exceed LDS ([3, 200003])
-    for dims, perm, elem in (([9, 271, 362], [3, 2, 1], 8), ([3, 100003], [2, 1], 8),
-                             ([100003, 3], [2, 1], 8), ([2, 50021, 2], [2, 1, 3], 8),
-                             ([3, 200003], [2, 1], 8)):
-        got, want, peak = run_case(torch_cuda, dims, perm, elem, seed=6, alpha=0.05)
-        assert_bitexact(got, want, f"{dims} {perm}")
+    import eithot_amd as E
+    old = E.set_tuning("cycle_min_cell", 0)  # cycle following would take the large cells
+    try:
+        for dims, perm, elem in (([9, 271, 362], [3, 2, 1], 8), ([3, 100003], [2, 1], 8),
+                                 ([100003, 3], [2, 1], 8), ([2, 50021, 2], [2, 1, 3], 8),
+                                 ([3, 200003], [2, 1], 8)):
+            got, want, peak = run_case(torch_cuda, dims, perm, elem, seed=6, alpha=0.05)
+            assert_bitexact(got, want, f"{dims} {perm}")
+            assert peak <= 0.05 * got.nbytes
+    finally:
+        E.set_tuning("cycle_min_cell", old)
+
+
+def test_cycle_following_passes(torch_cuda):
+    # tile transposes of large cells move whole cells along the cycles of the
+    # transpose: whole cycles, cut cycles (boundary slots), partial chunks,
+    # 16/8/4/2/1-byte words
+    import eithot_amd as E
+    cases = [([128, 8, 36, 5], [1, 3, 2, 4], 4),      # (5, 36x8 cells of 512 B), whole cycles
+             ([1536, 64, 120], [1, 3, 2], 4),         # one 120x64 matrix of 6 KB cells
+             ([96, 48, 320, 3], [1, 3, 2, 4], 8),     # 48x320 of 768 B
+             ([999, 17, 29], [1, 3, 2], 1),           # odd cells: 1-byte words
+             ([257, 20, 9], [1, 3, 2], 2),            # 514-byte cells, 2-byte words
+             ([300, 40, 7, 2], [1, 3, 2, 4], 2),      # 600-byte cells, 8-byte words
+             ([129, 33, 31], [1, 3, 2], 4)]           # 516-byte cells, 4-byte words
+    for dims, perm, elem in cases:
+        ops, _ = E.plan_describe(dims, perm, elem, 1, 0.05)
+        assert any(o["kind"] == 4 for o in ops), (dims, ops)
+        got, want, peak = run_case(torch_cuda, dims, perm, elem, seed=8, alpha=0.05)
+        assert_bitexact(got, want, f"{dims} {perm} elem {elem}")
         assert peak <= 0.05 * got.nbytes
 
 

@@ -109,7 +109,12 @@ NARROW_CASES = [([9, 271, 362], [3, 2, 1], 8, 0.05, 0), ([3, 100003], [2, 1], 8,
 def test_narrow_slice_plans_execute_correctly(dims, perm1, elem, alpha, lds):
     # rows too long for LDS at an aligned slice and for the scratch budget as
     # whole units: the row pass moves narrow slices of each unit through LDS
-    ops, scratch = E.plan_describe(dims, perm1, elem, 1, alpha, lds)
+    # (cycle following, which would move the large cells instead, is off)
+    old = E.set_tuning("cycle_min_cell", 0)
+    try:
+        ops, scratch = E.plan_describe(dims, perm1, elem, 1, alpha, lds)
+    finally:
+        E.set_tuning("cycle_min_cell", old)
     narrow = [o for o in ops if o["kind"] == 3 and o["us"] < o["U"]]
     assert narrow, ops
     # staged only when 1-byte slices of a row exceed LDS
@@ -141,7 +146,7 @@ def test_config_plans():
     assert scratch == 0
     # config 2: one pass per reference stage, scratch within 5 % of 4.53 GB
     ops, scratch = E.plan_describe([32, 48, 64, 40, 36, 8], [6, 4, 2, 5, 1, 3], 4, 1, 0.05)
-    assert len(ops) <= 5 and all(o["kind"] == 1 for o in ops)
+    assert len(ops) <= 5 and all(o["kind"] in (1, 4) for o in ops)
     assert scratch <= 0.05 * 32 * 48 * 64 * 40 * 36 * 8 * 4
     # config 3: scratch within 5 % of 171.8 GB
     ops, scratch = E.plan_describe([2, 163840, 65536, 2], [1, 3, 2, 4], 4, 1, 0.05)
