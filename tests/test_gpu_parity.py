@@ -70,7 +70,7 @@ def test_cycle_following_passes(torch_cuda):
              ([1536, 64, 120], [1, 3, 2], 4),         # one 120x64 matrix of 6 KB cells
              ([96, 48, 320, 3], [1, 3, 2, 4], 8),     # 48x320 of 768 B
              ([999, 17, 29], [1, 3, 2], 1),           # odd cells: 1-byte words
-             ([257, 20, 9], [1, 3, 2], 2),            # 514-byte cells, 2-byte words
+             ([257, 40, 9], [1, 3, 2], 2),            # 514-byte cells, 2-byte words
              ([300, 40, 7, 2], [1, 3, 2, 4], 2),      # 600-byte cells, 8-byte words
              ([129, 33, 31], [1, 3, 2], 4)]           # 516-byte cells, 4-byte words
     for dims, perm, elem in cases:
