@@ -526,7 +526,7 @@ struct Lowerer {
       t = build_cycle_table(R, C, lmax);
     }
     const int64_t N = R * C;
-    while (!t->slots.empty() && B * (int64_t)t->slots.size() * U > scratch_budget) {
+    while (!t->slots.empty() && B * (int64_t)t->slots.size() * U + t->bytes() > scratch_budget) {
       lmax = lmax >= N ? N : lmax * 2;
       t = build_cycle_table(R, C, lmax);
     }
@@ -790,10 +790,20 @@ int make_plan(const int64_t* dims_in, const int* perm1, int rank, size_t elem, i
       L.emit(st.S, st.Y, st.X, st.P * (int64_t)elem, 0, (int)s, 0, plan.ops);
     }
   }
+  int64_t fp_max = 0;
   for (const Op& op : plan.ops) {
     if (op.kind == 0) return EITHOT_ERR_ALPHA;
     plan.scratch_bytes = std::max(plan.scratch_bytes, staged_scratch(op, budget, prof));
+    if (op.staged) fp_max = std::max(fp_max, op.footprint_bytes());
     if (op.cyc) plan.table_bytes += op.cyc->bytes();
+  }
+  // the cycle tables count as extra memory too: without cycle following if
+  // they would push the plan over its budget
+  if (plan.table_bytes > 0 && plan.scratch_bytes + plan.table_bytes > std::max(budget, fp_max) &&
+      prof.cycle_min_cell > 0) {
+    DeviceProfile p2 = prof;
+    p2.cycle_min_cell = 0;
+    return make_plan(dims_in, perm1, rank, elem, n_sub, alpha, p2, reorder, plan);
   }
   return EITHOT_OK;
 }
@@ -1019,9 +1029,9 @@ int estimate_native(const int64_t* dims, const int* perm1, int rank, size_t elem
   int rc = make_plan(dims, perm1, rank, elem, 1, alpha, prof, prof.reorder, plan);
   if (rc != EITHOT_OK) return rc;
   if (n_sub) *n_sub = plan.n_sub;
-  if (extra_bytes) *extra_bytes = (size_t)plan.scratch_bytes;
+  if (extra_bytes) *extra_bytes = (size_t)(plan.scratch_bytes + plan.table_bytes);
   if (alpha_out)
-    *alpha_out = plan.tensor_bytes ? (double)plan.scratch_bytes / (double)plan.tensor_bytes : 0.0;
+    *alpha_out = plan.tensor_bytes ? (double)(plan.scratch_bytes + plan.table_bytes) / (double)plan.tensor_bytes : 0.0;
   return EITHOT_OK;
 }
 

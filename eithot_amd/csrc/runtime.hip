@@ -345,7 +345,7 @@ int eithot_plan_describe(const int64_t* dims, const int* perm, int order, size_t
   Plan plan;
   int rc = make_plan(dims, perm, order, elem_bytes, n_sub, alpha, prof, prof.reorder, plan);
   if (rc) return fail(rc, "planning failed (%d)", rc);
-  if (scratch_bytes) *scratch_bytes = plan.scratch_bytes;
+  if (scratch_bytes) *scratch_bytes = plan.scratch_bytes + plan.table_bytes;
   const int n = (int)plan.ops.size();
   for (int i = 0; i < n && i < max_ops && ops; ++i) {
     const Op& o = plan.ops[i];

@@ -107,7 +107,7 @@ int eithot_decompose(const int64_t* dims, const int* perm, int order, int reorde
 
 /* One device pass of a plan, for introspection and tests. */
 typedef struct {
-  int kind;      /* 1 tile, 2 column, 3 row */
+  int kind;      /* 1 tile, 2 column, 3 row, 4 tile by cycle following */
   int map;       /* 0 none, 1 rot, 2 rotinv, 3 colsh, 4 colshinv, 5 rowg, 6 rowd */
   int staged;    /* 1 when executed through scratch */
   int stage;     /* decomposer stage, -1 for partition/join */
@@ -118,7 +118,8 @@ typedef struct {
 
 /* Fills up to max_ops pass descriptors of the plan for the gfx950 profile
  * (lds_max bytes of LDS per workgroup; <= 0 selects 160 KiB) and returns
- * the pass count; *scratch_bytes receives the plan's scratch need. */
+ * the pass count; *scratch_bytes receives the plan's extra device memory
+ * (scratch plus the cycle tables of its cycle-following passes). */
 int eithot_plan_describe(const int64_t* dims, const int* perm, int order, size_t elem_bytes,
                          int n_sub, double alpha, int64_t lds_max, eithot_op_desc* ops,
                          int max_ops, int64_t* scratch_bytes);
