@@ -1147,6 +1147,91 @@ __global__ __launch_bounds__(256) void k_cycle(CycArgs a) {
   }
 }
 
+// Pipelined variant: every lane moves V words per position (a wave covers
+// V*64 words of the cell, each load one coalesced row of the wave), and the
+// loads of the next D positions are issued before the stores of the current
+// D, so up to 2*D*V loads per lane are in flight.  Hazards as above: a
+// store to a position follows the completed load of that position's old
+// cell, and loads of batch t+1 never touch positions stored in batch t.
+template <typename Word, int D, int V>
+__global__ __launch_bounds__(256) void k_cycle_pipe(CycArgs a) {
+  constexpr uint64_t w = sizeof(Word);
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t item = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (item >= a.items) return;
+  const uint64_t rest = a.fchunks.div(item);
+  const uint64_t chunk = item - rest * a.chunks;
+  const uint64_t b = a.fnseg.div(rest);
+  const uint64_t s = rest - b * a.nseg;
+  const uint64_t off0 = chunk * a.chunk_bytes + lane * w;
+  if (off0 >= a.U) return;
+  bool on[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) on[v] = off0 + (uint64_t)v * 64 * w < a.U;
+  char* m = a.data + a.base + b * a.RCU + off0;
+  const CycleSeg sg = a.segs[s];
+  const char* tsrc = sg.src_slot < 0
+                         ? m + (uint64_t)sg.start * a.U
+                         : a.boundary + (b * a.nslots + (uint64_t)sg.src_slot) * a.U + off0;
+  Word temp[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v)
+    if (on[v]) temp[v] = *reinterpret_cast<const Word*>(tsrc + v * 64 * w);
+  auto next = [&](uint32_t pp) {
+    const uint32_t j = a.fR.div(pp);
+    return (pp - j * a.R) * a.C + j;
+  };
+  Word cur[D][V], nxt[D][V];
+  uint32_t qc[D], qn[D];
+  uint32_t p = sg.start;
+  // batch 0: old cells of positions 1..D
+  {
+    uint32_t pp = p;
+#pragma unroll
+    for (int k = 0; k < D; ++k) qc[k] = pp = next(pp);
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        if ((uint32_t)k < sg.len && on[v])
+          cur[k][v] = (uint32_t)k + 1 == sg.len
+                          ? temp[v]
+                          : *reinterpret_cast<const Word*>(m + (uint64_t)qc[k] * a.U + v * 64 * w);
+  }
+  for (uint32_t t = 0; t < sg.len; t += D) {
+    // loads of batch t + D
+    uint32_t pp = qc[D - 1];
+#pragma unroll
+    for (int k = 0; k < D; ++k) qn[k] = pp = next(pp);
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        if (t + D + k < sg.len && on[v])
+          nxt[k][v] = t + D + k + 1 == sg.len
+                          ? temp[v]
+                          : *reinterpret_cast<const Word*>(m + (uint64_t)qn[k] * a.U + v * 64 * w);
+    // stores of batch t
+    uint32_t dst = p;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      if (t + k < sg.len) {
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+          if (on[v]) *reinterpret_cast<Word*>(m + (uint64_t)dst * a.U + v * 64 * w) = cur[k][v];
+      }
+      dst = qc[k];
+    }
+    p = qc[D - 1];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      qc[k] = qn[k];
+#pragma unroll
+      for (int v = 0; v < V; ++v) cur[k][v] = nxt[k][v];
+    }
+  }
+}
+
 const void* cycle_table_device(const CycleTable& t, hipError_t* err) {
   *err = hipSuccess;
   int dev = 0;
@@ -1168,19 +1253,29 @@ const void* cycle_table_device(const CycleTable& t, hipError_t* err) {
 }
 
 template <typename Word>
-hipError_t launch_cycle_w(CycArgs a, hipStream_t stream) {
+hipError_t launch_cycle_w(CycArgs a, int variant, hipStream_t stream) {
   if (a.nslots) {
     const uint64_t words = a.B * a.nslots * (a.U / sizeof(Word));
     const uint64_t blocks = std::min<uint64_t>((words + 255) / 256, 256ull * 64);
     hipLaunchKernelGGL((k_cycle_save<Word>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
   }
   const uint64_t blocks = (a.items + 3) / 4;
-  hipLaunchKernelGGL((k_cycle<Word, 8>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
+  if (variant == 1)
+    hipLaunchKernelGGL((k_cycle_pipe<Word, 4, 2>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
+  else if (variant == 2)
+    hipLaunchKernelGGL((k_cycle_pipe<Word, 4, 1>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
+  else if (variant == 3)
+    hipLaunchKernelGGL((k_cycle_pipe<Word, 2, 4>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
+  else
+    hipLaunchKernelGGL((k_cycle<Word, 8>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 
+// lanes' words per position of each cycle kernel variant
+inline int cycle_vwords(int variant) { return variant == 1 ? 2 : variant == 3 ? 4 : 1; }
+
 hipError_t launch_cycle(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
-                        hipStream_t stream) {
+                        int variant, hipStream_t stream) {
   if (!op.cyc || (uint64_t)op.R * (uint64_t)op.C >= (1ull << 32)) return hipErrorInvalidValue;
   const CycleTable& t = *op.cyc;
   if (t.segs.empty()) return hipSuccess;  // only fixed points
@@ -1202,7 +1297,7 @@ hipError_t launch_cycle(const Op& op, char* data, char* scratch, int64_t scratch
   a.nslots = (uint32_t)t.slots.size();
   a.R = (uint32_t)op.R;
   a.C = (uint32_t)op.C;
-  a.chunk_bytes = (uint32_t)(64 * w);
+  a.chunk_bytes = (uint32_t)(64 * w * cycle_vwords(variant));
   a.B = op.B;
   a.chunks = (op.U + a.chunk_bytes - 1) / a.chunk_bytes;
   a.items = a.B * a.nseg * a.chunks;
@@ -1211,11 +1306,11 @@ hipError_t launch_cycle(const Op& op, char* data, char* scratch, int64_t scratch
   a.fnseg.init(a.nseg);
   if ((a.items + 3) / 4 >= (1ull << 31)) return hipErrorInvalidValue;
   switch (w) {
-    case 16: return launch_cycle_w<W16>(a, stream);
-    case 8: return launch_cycle_w<uint64_t>(a, stream);
-    case 4: return launch_cycle_w<uint32_t>(a, stream);
-    case 2: return launch_cycle_w<uint16_t>(a, stream);
-    default: return launch_cycle_w<uint8_t>(a, stream);
+    case 16: return launch_cycle_w<W16>(a, variant, stream);
+    case 8: return launch_cycle_w<uint64_t>(a, variant, stream);
+    case 4: return launch_cycle_w<uint32_t>(a, variant, stream);
+    case 2: return launch_cycle_w<uint16_t>(a, variant, stream);
+    default: return launch_cycle_w<uint8_t>(a, variant, stream);
   }
 }
 
@@ -1230,7 +1325,8 @@ hipError_t prepare_op(const Op& op) {
 
 hipError_t launch_op(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
                      const LaunchCfg& cfg, hipStream_t stream) {
-  if (op.kind == OP_CYCLE) return launch_cycle(op, data, scratch, scratch_bytes, stream);
+  if (op.kind == OP_CYCLE)
+    return launch_cycle(op, data, scratch, scratch_bytes, cfg.cycle_variant, stream);
   if (op.kind == 0 || op.us <= 0 || op.U % op.us) return hipErrorInvalidValue;
   if (needs64(op)) return run_op<uint64_t>(op, data, scratch, scratch_bytes, cfg, stream);
   return run_op<uint32_t>(op, data, scratch, scratch_bytes, cfg, stream);

@@ -16,6 +16,7 @@ struct LaunchCfg {
   int threads = 256;
   int threads_big = 512;           // workgroups whose LDS footprint exceeds 96 KiB
   int xcd_remap = 1;               // keep consecutive footprints on one XCD
+  int cycle_variant = 0;           // cycle-following kernel shape (0..3)
 };
 
 // One pass of `op` over the tensor at `data`, on `stream`.  Staged ops use

@@ -420,18 +420,21 @@ enum How {
 };
 
 // Relative time of one pass over the tensor, by pass class (contiguous LDS
-// tile = 1), from MI355X measurements: sliced tiles move short scattered
-// segments (about 2.3x a contiguous tile), staged passes cross HBM twice;
-// Catanzaro passes evaluate heavier index maps.
+// tile = 1, 5.7 TB/s of reads + writes on MI355X), from measurements on
+// 4.5 GB tensors: sliced LDS tiles move short scattered segments (2.3x),
+// staged tiles cross HBM twice (2.05x, 2.4x when sliced), cycle following
+// reads and writes each cell once at scattered positions (1.6x, 1.75x for
+// cells under 1 KiB); Catanzaro passes evaluate heavier index maps.
 constexpr double kCostTile = 1.0;
-constexpr double kCostTileSliced = 2.2;
-constexpr double kCostTileStaged = 2.1;
+constexpr double kCostTileSliced = 2.3;
+constexpr double kCostTileStaged = 2.05;
+constexpr double kCostTileStagedSliced = 2.4;
 constexpr double kCostSplit = 0.001;
 
 // Cycle-following moves whole cells (of at least cycle_min_cell bytes), one
 // read and one write each; smaller cells waste HBM bursts.
 constexpr uint64_t kCycleMaxCells = 1ull << 26;
-inline double cycle_cost(int64_t U) { return 1.05 + 256.0 / (double)U; }
+inline double cycle_cost(int64_t U) { return U >= 1024 ? 1.6 : 1.75; }
 
 struct Choice {
   int how = HOW_NONE;
@@ -479,9 +482,10 @@ struct Lowerer {
     // whole matrices (or slices) through scratch: copy out, tiled transpose back
     if (ch.cost > kCostTileStaged) {
       const int64_t us = pick_staged_tile_slice(R, C, U, scratch_budget, prof);
-      if (us) {
+      const double cost = us == U ? kCostTileStaged : kCostTileStagedSliced;
+      if (us && cost < ch.cost) {
         ch.how = HOW_TILE_STAGED;
-        ch.cost = kCostTileStaged;
+        ch.cost = cost;
       }
     }
     if (depth < 3 && ch.cost > 2 * kCostTile + 1e-9) {
@@ -671,6 +675,7 @@ static int64_t* tuning_field(Tuning& t, const char* key) {
   if (!strcmp(key, "threads")) return &t.threads;
   if (!strcmp(key, "threads_big")) return &t.threads_big;
   if (!strcmp(key, "xcd_remap")) return &t.xcd_remap;
+  if (!strcmp(key, "cycle_variant")) return &t.cycle_variant;
   if (!strcmp(key, "reorder")) return &t.reorder;
   return nullptr;
 }
@@ -683,6 +688,7 @@ int tuning_set(const char* key, int64_t value) {
   const bool threads = f == &t.threads || f == &t.threads_big;
   if (threads && (value < 64 || value > 1024 || value % 64)) return EITHOT_ERR_INVALID;
   if (f == &t.reorder && value > 2) return EITHOT_ERR_INVALID;
+  if (f == &t.cycle_variant && value > 3) return EITHOT_ERR_INVALID;
   if (f == &t.lds_target && value < 1024) return EITHOT_ERR_INVALID;
   *f = value;
   t.epoch = g_tune.epoch + 1;

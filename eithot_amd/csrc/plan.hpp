@@ -120,6 +120,7 @@ struct Tuning {
   int64_t threads = 256;
   int64_t threads_big = 512;
   int64_t xcd_remap = 1;
+  int64_t cycle_variant = 0;
   int64_t reorder = 0;
   uint64_t epoch = 0;  // bumped on every change (plan caches key on it)
 };

@@ -89,6 +89,7 @@ LaunchCfg Device::launch_cfg() const {
   c.threads = (int)t.threads;
   c.threads_big = (int)t.threads_big;
   c.xcd_remap = (int)t.xcd_remap;
+  c.cycle_variant = (int)t.cycle_variant;
   return c;
 }
 

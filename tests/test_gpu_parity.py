@@ -73,12 +73,18 @@ def test_cycle_following_passes(torch_cuda):
              ([257, 40, 9], [1, 3, 2], 2),            # 514-byte cells, 2-byte words
              ([300, 40, 7, 2], [1, 3, 2, 4], 2),      # 600-byte cells, 8-byte words
              ([129, 33, 31], [1, 3, 2], 4)]           # 516-byte cells, 4-byte words
-    for dims, perm, elem in cases:
-        ops, _ = E.plan_describe(dims, perm, elem, 1, 0.05)
-        assert any(o["kind"] == 4 for o in ops), (dims, ops)
-        got, want, peak = run_case(torch_cuda, dims, perm, elem, seed=8, alpha=0.05)
-        assert_bitexact(got, want, f"{dims} {perm} elem {elem}")
-        assert peak <= 0.05 * got.nbytes
+    default = E.get_tuning("cycle_variant")
+    try:
+        for variant in (0, 1, 2, 3):  # every kernel shape, whichever is the default
+            E.set_tuning("cycle_variant", variant)
+            for dims, perm, elem in cases:
+                ops, _ = E.plan_describe(dims, perm, elem, 1, 0.05)
+                assert any(o["kind"] == 4 for o in ops), (dims, ops)
+                got, want, peak = run_case(torch_cuda, dims, perm, elem, seed=8, alpha=0.05)
+                assert_bitexact(got, want, f"variant {variant} {dims} {perm} elem {elem}")
+                assert peak <= 0.05 * got.nbytes
+    finally:
+        E.set_tuning("cycle_variant", default)
 
 
 def test_random_high_order(torch_cuda):
