@@ -36,8 +36,18 @@ def inverse(dims, perm1):
     return out_dims, inv
 
 
+# HIP kernels that carry each pass class (staged passes launch two per group)
+HIP_KERNEL = {"cycle": "k_cycle", "tile": "k_tile", "tile_staged": "copy + k_tile_oop"}
+for _m in ("_rot", "_rotinv", "_colsh", "_colshinv"):
+    HIP_KERNEL["col" + _m] = "k_col"
+    HIP_KERNEL["col" + _m + "_staged"] = "k_stage_out + k_stage_in"
+for _m in ("_rowg", "_rowd"):
+    HIP_KERNEL["row" + _m] = "k_row"
+    HIP_KERNEL["row" + _m + "_staged"] = "k_stage_out + k_stage_in"
+
+
 def kernel_name(op):
-    kinds = {1: "tile", 2: "col", 3: "row"}
+    kinds = {1: "tile", 2: "col", 3: "row", 4: "cycle"}
     maps = {0: "", 1: "_rot", 2: "_rotinv", 3: "_colsh", 4: "_colshinv", 5: "_rowg", 6: "_rowd"}
     return f"{kinds[op['kind']]}{maps[op['map']]}{'_staged' if op['staged'] else ''}"
 
@@ -445,7 +455,8 @@ def main():
                    "peak_extra_bytes": peak,
                    "passes_per_step": {"forward": len(pass_ms[0][0]), "inverse": len(pass_ms[1][0])},
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
+        "roofline": {"bound": "hbm", "kernel": dom, "hip_kernel": HIP_KERNEL.get(dom, dom),
+                     "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "per_kernel": {k: {"ms_per_step": round(v["ms"] / 2, 4),

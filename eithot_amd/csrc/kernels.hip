@@ -1102,7 +1102,41 @@ __global__ __launch_bounds__(256) void k_cycle_save(CycArgs a) {
   }
 }
 
-template <typename Word, int D>
+// global accesses with optional non-temporal hints (NT bit 0: loads, bit 1: stores)
+typedef uint32_t U32x4 __attribute__((ext_vector_type(4)));
+template <int NT, typename Word>
+__device__ __forceinline__ Word ld_nt(const char* p) {
+  if constexpr ((NT & 1) && sizeof(Word) == 16) {
+    const U32x4 v = __builtin_nontemporal_load(reinterpret_cast<const U32x4*>(p));
+    Word w;
+    w.v[0] = v.x;
+    w.v[1] = v.y;
+    w.v[2] = v.z;
+    w.v[3] = v.w;
+    return w;
+  } else if constexpr ((NT & 1) && sizeof(Word) < 16) {
+    return __builtin_nontemporal_load(reinterpret_cast<const Word*>(p));
+  } else {
+    return *reinterpret_cast<const Word*>(p);
+  }
+}
+template <int NT, typename Word>
+__device__ __forceinline__ void st_nt(char* p, const Word& w) {
+  if constexpr ((NT & 2) && sizeof(Word) == 16) {
+    U32x4 v;
+    v.x = w.v[0];
+    v.y = w.v[1];
+    v.z = w.v[2];
+    v.w = w.v[3];
+    __builtin_nontemporal_store(v, reinterpret_cast<U32x4*>(p));
+  } else if constexpr ((NT & 2) && sizeof(Word) < 16) {
+    __builtin_nontemporal_store(w, reinterpret_cast<Word*>(p));
+  } else {
+    *reinterpret_cast<Word*>(p) = w;
+  }
+}
+
+template <typename Word, int D, int NT = 0>
 __global__ __launch_bounds__(256) void k_cycle(CycArgs a) {
   constexpr uint64_t w = sizeof(Word);
   const uint32_t lane = threadIdx.x & 63;
@@ -1135,12 +1169,12 @@ __global__ __launch_bounds__(256) void k_cycle(CycArgs a) {
 #pragma unroll
     for (int k = 0; k < D; ++k)
       if (t + k < sg.len)
-        v[k] = t + k + 1 == sg.len ? temp : *reinterpret_cast<const Word*>(m + (uint64_t)q[k] * a.U);
+        v[k] = t + k + 1 == sg.len ? temp : ld_nt<NT, Word>(m + (uint64_t)q[k] * a.U);
     uint32_t dst = p;
 #pragma unroll
     for (int k = 0; k < D; ++k)
       if (t + k < sg.len) {
-        *reinterpret_cast<Word*>(m + (uint64_t)dst * a.U) = v[k];
+        st_nt<NT, Word>(m + (uint64_t)dst * a.U, v[k]);
         dst = q[k];
       }
     p = q[D - 1];
@@ -1266,6 +1300,14 @@ hipError_t launch_cycle_w(CycArgs a, int variant, hipStream_t stream) {
     hipLaunchKernelGGL((k_cycle_pipe<Word, 4, 1>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
   else if (variant == 3)
     hipLaunchKernelGGL((k_cycle_pipe<Word, 2, 4>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
+  else if (variant == 4)
+    hipLaunchKernelGGL((k_cycle<Word, 8, 2>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
+  else if (variant == 5)
+    hipLaunchKernelGGL((k_cycle<Word, 8, 3>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
+  else if (variant == 6)
+    hipLaunchKernelGGL((k_cycle<Word, 8, 1>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
+  else if (variant == 7)
+    hipLaunchKernelGGL((k_cycle<Word, 16, 0>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
   else
     hipLaunchKernelGGL((k_cycle<Word, 8>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
   return hipGetLastError();

@@ -688,7 +688,7 @@ int tuning_set(const char* key, int64_t value) {
   const bool threads = f == &t.threads || f == &t.threads_big;
   if (threads && (value < 64 || value > 1024 || value % 64)) return EITHOT_ERR_INVALID;
   if (f == &t.reorder && value > 2) return EITHOT_ERR_INVALID;
-  if (f == &t.cycle_variant && value > 3) return EITHOT_ERR_INVALID;
+  if (f == &t.cycle_variant && value > 7) return EITHOT_ERR_INVALID;
   if (f == &t.lds_target && value < 1024) return EITHOT_ERR_INVALID;
   *f = value;
   t.epoch = g_tune.epoch + 1;

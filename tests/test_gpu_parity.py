@@ -75,7 +75,7 @@ def test_cycle_following_passes(torch_cuda):
              ([129, 33, 31], [1, 3, 2], 4)]           # 516-byte cells, 4-byte words
     default = E.get_tuning("cycle_variant")
     try:
-        for variant in (0, 1, 2, 3):  # every kernel shape, whichever is the default
+        for variant in range(8):  # every kernel shape, whichever is the default
             E.set_tuning("cycle_variant", variant)
             for dims, perm, elem in cases:
                 ops, _ = E.plan_describe(dims, perm, elem, 1, 0.05)
