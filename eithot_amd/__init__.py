@@ -117,7 +117,15 @@ def plan_describe(dims, perm1, elem_bytes, n_sub=1, alpha=0.05, lds_max=0):
     out = []
     for i in range(min(n, 512)):
         o = ops[i]
-        out.append({f: getattr(o, f) for f, _ in _lib.OpDesc._fields_})
+        d = {f: getattr(o, f) for f, _ in _lib.OpDesc._fields_}
+        if d["kind"] == 5:  # fused block permutation: attach its gather table
+            import numpy as np
+            tab = np.zeros(d["R"], dtype=np.uint32)
+            check(lib().eithot_plan_block_table(
+                i64(dims), i32(perm1), len(dims), int(elem_bytes), int(n_sub), float(alpha),
+                int(lds_max), i, tab.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), tab.size))
+            d["table"] = tab
+        out.append(d)
     return out, scratch.value
 
 

@@ -1054,6 +1054,7 @@ __global__ void k_verify(const T* p, VerifyArgs v, uint64_t seed_mixed, int mode
 
 int64_t op_lds_bytes(const Op& op, const LaunchCfg& cfg) {
   if (op.staged || op.kind == 0 || op.kind == OP_CYCLE) return 0;
+  if (op.kind == OP_BLOCK) return op.blk ? (op.blk->lds_bytes() + 15) / 16 * 16 : op.R * op.U;
   Args<uint64_t> a;
   Prepared p;
   prepare_lds<uint64_t>(op, nullptr, cfg, a, p);
@@ -1356,12 +1357,187 @@ hipError_t launch_cycle(const Op& op, char* data, char* scratch, int64_t scratch
   }
 }
 
+// ---------------------------------------------------------------------------
+// fused block permutation (OP_BLOCK): K contiguous blocks of n cells per
+// workgroup are loaded into LDS with 16-byte loads, then every output cell
+// o takes LDS cell T[o] of its block (T: uint16 indices when n <= 65536).
+// Small cells (one word each) are packed V to a 16-byte store.
+// ---------------------------------------------------------------------------
+struct BlkArgs {
+  char* data;
+  uint64_t base, F, B;  // F: bytes per block
+  const void* table;
+  uint32_t n, wpc, K, vec16, idx16, chunks;  // chunks: n / V
+  uint32_t pad, Fl;                           // pad: log2 of the pad interval (0: none); Fl: LDS bytes per block
+  FD32 fnw, fwpc, fch, fF16, fFw;             // n * wpc, wpc, chunks, F / 16, F / w
+};
+
+// LDS byte offset of byte x of a block (4 pad bytes after every 2^pad bytes)
+__device__ __forceinline__ uint32_t lds_off(uint32_t x, uint32_t pad) {
+  return pad ? x + ((x >> pad) << 2) : x;
+}
+
+template <typename Word, int V>
+__global__ __launch_bounds__(256) void k_block(BlkArgs a, int remap) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  constexpr uint32_t w = sizeof(Word);
+  const uint64_t f0 = (uint64_t)xcd_logical(blockIdx.x, gridDim.x, remap) * a.K;
+  const uint32_t nf = (uint32_t)((a.B - f0) < a.K ? (a.B - f0) : a.K);
+  const uint32_t tid = threadIdx.x, nt = blockDim.x;
+  const char* g = a.data + a.base + f0 * a.F;
+  const uint32_t c16 = (uint32_t)(a.F / 16);
+  if (a.vec16) {
+    const uint32_t n16 = nf * c16;
+    for (uint32_t t0 = tid; t0 < n16; t0 += nt * UNR) {
+      W16 v[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const uint32_t t = t0 + u * nt;
+        if (t < n16) v[u] = reinterpret_cast<const W16*>(g)[t];
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const uint32_t t = t0 + u * nt;
+        if (t < n16) {
+          const uint32_t k = a.fF16.div(t);
+          lds_put16(lds + k * a.Fl + lds_off((t - k * c16) * 16, a.pad), v[u]);
+        }
+      }
+    }
+  } else {
+    const uint32_t fw = (uint32_t)(a.F / w);
+    const uint32_t nw = nf * fw;
+    for (uint32_t t = tid; t < nw; t += nt) {
+      const uint32_t k = a.fFw.div(t);
+      *reinterpret_cast<Word*>(lds + k * a.Fl + lds_off((t - k * fw) * w, a.pad)) =
+          reinterpret_cast<const Word*>(g)[t];
+    }
+  }
+  __syncthreads();
+  const uint16_t* T16 = reinterpret_cast<const uint16_t*>(a.table);
+  const uint32_t* T32 = reinterpret_cast<const uint32_t*>(a.table);
+  char* out = a.data + a.base + f0 * a.F;
+  if (V > 1) {
+    // one word per cell, V cells per 16-byte store
+    const uint32_t total = nf * a.chunks;
+    for (uint32_t t = tid; t < total; t += nt) {
+      const uint32_t k = a.fch.div(t);
+      const uint32_t o0 = (t - k * a.chunks) * V;
+      Word tmp[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const uint32_t src = a.idx16 ? (uint32_t)T16[o0 + v] : T32[o0 + v];
+        tmp[v] = *reinterpret_cast<const Word*>(lds + k * a.Fl + lds_off(src * w, a.pad));
+      }
+      *reinterpret_cast<Vec<V * sizeof(Word)>*>(out + k * a.F + (uint64_t)o0 * w) =
+          *reinterpret_cast<const Vec<V * sizeof(Word)>*>(tmp);
+    }
+  } else {
+    const uint32_t per = a.n * a.wpc;
+    const uint32_t total = nf * per;
+    for (uint32_t t = tid; t < total; t += nt) {
+      const uint32_t k = a.fnw.div(t);
+      const uint32_t r = t - k * per;
+      const uint32_t o = a.wpc == 1 ? r : a.fwpc.div(r);
+      const uint32_t x = r - o * a.wpc;
+      const uint32_t src = a.idx16 ? (uint32_t)T16[o] : T32[o];
+      reinterpret_cast<Word*>(out)[(uint64_t)k * per + r] = *reinterpret_cast<const Word*>(
+          lds + k * a.Fl + lds_off((src * a.wpc + x) * w, a.pad));
+    }
+  }
+}
+
+const void* block_table_device(const BlockTable& t, hipError_t* err) {
+  *err = hipSuccess;
+  int dev = 0;
+  if ((*err = hipGetDevice(&dev)) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(t.dev_mu);
+  auto it = t.dev.find(dev);
+  if (it != t.dev.end()) return it->second.get();
+  const bool i16 = t.cells <= 65536;
+  std::vector<uint16_t> h16;
+  if (i16) h16.assign(t.src.begin(), t.src.end());
+  const size_t nb = t.src.size() * (i16 ? 2 : 4);
+  void* p = nullptr;
+  if ((*err = hipMalloc(&p, std::max<size_t>(nb, 16))) != hipSuccess) return nullptr;
+  std::shared_ptr<void> own(p, [](void* q) { (void)hipFree(q); });
+  if ((*err = hipMemcpy(p, i16 ? (const void*)h16.data() : (const void*)t.src.data(), nb,
+                        hipMemcpyHostToDevice)) != hipSuccess)
+    return nullptr;
+  t.dev[dev] = own;
+  return p;
+}
+
+template <typename Word>
+hipError_t launch_block_w(BlkArgs a, int vec, int64_t lds, int64_t groups, const LaunchCfg& cfg,
+                          hipStream_t stream) {
+  constexpr int V = 16 / sizeof(Word) > 1 ? 16 / sizeof(Word) : 1;
+  const int threads = lds > 96 * 1024 ? cfg.threads_big : cfg.threads;
+  if (vec && V > 1)
+    hipLaunchKernelGGL((k_block<Word, V>), dim3((uint32_t)groups), dim3(threads), (size_t)lds, stream,
+                       a, cfg.xcd_remap);
+  else
+    hipLaunchKernelGGL((k_block<Word, 1>), dim3((uint32_t)groups), dim3(threads), (size_t)lds, stream,
+                       a, cfg.xcd_remap);
+  return hipGetLastError();
+}
+
+hipError_t launch_block(const Op& op, char* data, const LaunchCfg& cfg, hipStream_t stream) {
+  if (!op.blk || op.blk->cells != op.R || op.blk->cell_bytes != op.U) return hipErrorInvalidValue;
+  const BlockTable& t = *op.blk;
+  const int64_t F = op.R * op.U;
+  if (F > 160 * 1024 || op.R >= (1ll << 31) || op.B <= 0) return hipErrorInvalidValue;
+  hipError_t e;
+  const void* d = block_table_device(t, &e);
+  if (!d) return e;
+  const uintptr_t base = (uintptr_t)(data + op.base);
+  int64_t w = 16;
+  while (w > 1 && ((op.U % w) || (base % w) || (F % w))) w /= 2;
+  BlkArgs a;
+  a.data = data;
+  a.base = op.base;
+  a.F = (uint64_t)F;
+  a.B = (uint64_t)op.B;
+  a.table = d;
+  a.n = (uint32_t)op.R;
+  a.wpc = (uint32_t)(op.U / w);
+  a.vec16 = (F % 16 == 0 && base % 16 == 0) ? 1 : 0;
+  a.idx16 = t.cells <= 65536 ? 1 : 0;
+  const int64_t V = w < 16 ? 16 / w : 1;
+  const int vec = V > 1 && a.wpc == 1 && op.R % V == 0 && a.vec16;
+  a.chunks = vec ? (uint32_t)(op.R / V) : (uint32_t)op.R;
+  // padding keeps cells of up to 16 bytes whole (pad intervals are >= 64 bytes)
+  // (and LDS words stay aligned: only 1-, 2- and 4-byte words are padded)
+  a.pad = (t.pad_log2 && 64 % op.U == 0 && w <= 4) ? (uint32_t)t.pad_log2 : 0;
+  const int64_t Fl = a.pad ? F + (F >> a.pad) * 4 + 4 : F;
+  a.Fl = (uint32_t)((Fl + 15) / 16 * 16);
+  if (a.Fl > 160 * 1024) return hipErrorInvalidValue;
+  int64_t K = pick_K(op.B, a.Fl, cfg.lds_target);
+  while (K > 1 && K * a.Fl > 160 * 1024) --K;
+  a.K = (uint32_t)K;
+  a.fnw.init(a.n * a.wpc);
+  a.fwpc.init(a.wpc);
+  a.fch.init(a.chunks);
+  a.fF16.init((uint32_t)std::max<int64_t>(F / 16, 1));
+  a.fFw.init((uint32_t)(F / w));
+  const int64_t groups = (op.B + K - 1) / K;
+  if (groups >= (1ll << 31)) return hipErrorInvalidValue;
+  const int64_t lds = K * (int64_t)a.Fl;
+  switch (w) {
+    case 16: return launch_block_w<W16>(a, vec, lds, groups, cfg, stream);
+    case 8: return launch_block_w<uint64_t>(a, vec, lds, groups, cfg, stream);
+    case 4: return launch_block_w<uint32_t>(a, vec, lds, groups, cfg, stream);
+    case 2: return launch_block_w<uint16_t>(a, vec, lds, groups, cfg, stream);
+    default: return launch_block_w<uint8_t>(a, vec, lds, groups, cfg, stream);
+  }
+}
+
 }  // namespace
 
 hipError_t prepare_op(const Op& op) {
-  if (op.kind != OP_CYCLE || !op.cyc || op.cyc->segs.empty()) return hipSuccess;
-  hipError_t e;
-  cycle_table_device(*op.cyc, &e);
+  hipError_t e = hipSuccess;
+  if (op.kind == OP_CYCLE && op.cyc && !op.cyc->segs.empty()) cycle_table_device(*op.cyc, &e);
+  if (op.kind == OP_BLOCK && op.blk) block_table_device(*op.blk, &e);
   return e;
 }
 
@@ -1369,6 +1545,7 @@ hipError_t launch_op(const Op& op, char* data, char* scratch, int64_t scratch_by
                      const LaunchCfg& cfg, hipStream_t stream) {
   if (op.kind == OP_CYCLE)
     return launch_cycle(op, data, scratch, scratch_bytes, cfg.cycle_variant, stream);
+  if (op.kind == OP_BLOCK) return launch_block(op, data, cfg, stream);
   if (op.kind == 0 || op.us <= 0 || op.U % op.us) return hipErrorInvalidValue;
   if (needs64(op)) return run_op<uint64_t>(op, data, scratch, scratch_bytes, cfg, stream);
   return run_op<uint32_t>(op, data, scratch, scratch_bytes, cfg, stream);
@@ -1396,7 +1573,7 @@ int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, cons
   bool big = false;
   for (size_t i = 0; i < ops.size(); ++i) {
     const Op& op = *ops[i].first;
-    if (op.staged || op.kind == 0 || op.kind == OP_CYCLE || needs64(op)) {
+    if (op.staged || op.kind == 0 || op.kind == OP_CYCLE || op.kind == OP_BLOCK || needs64(op)) {
       if (rejected) rejected->push_back(i);
       continue;
     }
@@ -1530,6 +1707,8 @@ void cfg_word(int64_t bytes, hipError_t& e) {
   };
   set((const void*)k_tile_oop<Word, uint32_t, 1>);
   set((const void*)k_tile_oop<Word, uint64_t, 1>);
+  set((const void*)k_block<Word, 1>);
+  if (V > 1) set((const void*)k_block<Word, V>);
 }
 }  // namespace
 

@@ -195,6 +195,7 @@ int64_t Op::footprint_units() const {
     case OP_COL: return R * W;
     case OP_ROW: return C;
     case OP_CYCLE: return 1;  // scratch: one saved cell per boundary slot
+    case OP_BLOCK: return R;
   }
   return 0;
 }
@@ -206,6 +207,7 @@ int64_t Op::num_footprints() const {
     case OP_COL: return B * ((C + W - 1) / W) * slices;
     case OP_ROW: return B * R * slices;
     case OP_CYCLE: return cyc ? B * (int64_t)cyc->slots.size() : 0;
+    case OP_BLOCK: return B;
   }
   return 0;
 }
@@ -672,6 +674,7 @@ static int64_t* tuning_field(Tuning& t, const char* key) {
   if (!strcmp(key, "slice_min")) return &t.slice_min;
   if (!strcmp(key, "stage_chunk")) return &t.stage_chunk;
   if (!strcmp(key, "cycle_min_cell")) return &t.cycle_min_cell;
+  if (!strcmp(key, "fuse_max_block")) return &t.fuse_max_block;
   if (!strcmp(key, "threads")) return &t.threads;
   if (!strcmp(key, "threads_big")) return &t.threads_big;
   if (!strcmp(key, "xcd_remap")) return &t.xcd_remap;
@@ -711,6 +714,7 @@ void tuning_apply(const Tuning& t, DeviceProfile& prof) {
   prof.slice_min = t.slice_min;
   prof.stage_chunk = t.stage_chunk;
   prof.cycle_min_cell = t.cycle_min_cell;
+  prof.fuse_max_block = std::min<int64_t>(t.fuse_max_block, prof.lds_max);
   prof.reorder = (int)t.reorder;
 }
 
@@ -731,8 +735,121 @@ static int64_t staged_scratch(const Op& op, int64_t budget, const DeviceProfile&
   return g * fp;
 }
 
-int make_plan(const int64_t* dims_in, const int* perm1, int rank, size_t elem, int n_sub,
-              double alpha, const DeviceProfile& prof, int reorder, Plan& plan) {
+// ---- fusion of nested contiguous tile passes -------------------------------------
+// Consecutive whole-unit tile passes over the same bytes whose matrices
+// (blocks of R*C*U bytes) nest form one permutation of every largest block;
+// when that block fits `fuse_max_block`, one LDS pass applies it.
+namespace {
+
+bool fusable_tile(const Op& o) { return o.kind == OP_TILE && !o.staged && o.us == o.U; }
+
+std::shared_ptr<BlockTable> compose_tiles(const std::vector<Op>& ops, size_t i0, size_t i1,
+                                          int64_t F, int64_t g) {
+  auto t = std::make_shared<BlockTable>();
+  const int64_t n = F / g;
+  t->cells = n;
+  t->cell_bytes = g;
+  std::vector<uint32_t> cur(n), nxt(n);
+  for (int64_t i = 0; i < n; ++i) cur[i] = (uint32_t)i;
+  for (size_t q = i0; q < i1; ++q) {
+    const Op& o = ops[q];
+    const int64_t k = o.U / g, R = o.R, C = o.C, fb = R * C * k;
+    for (int64_t b0 = 0; b0 < n; b0 += fb)
+      for (int64_t j = 0; j < C; ++j)
+        for (int64_t i = 0; i < R; ++i)
+          for (int64_t x = 0; x < k; ++x)
+            nxt[b0 + (j * R + i) * k + x] = cur[b0 + (i * C + j) * k + x];
+    cur.swap(nxt);
+  }
+  t->src = std::move(cur);
+  // LDS padding: the gather reads V = 16/g cells per thread (one word
+  // each) across 64 lanes; pick the pad interval with the fewest bank
+  // conflicts (64 banks of 4 bytes, and 32-lane halves on 32 banks)
+  if (g <= 16 && 64 % g == 0) {
+    const int64_t V = g < 16 ? 16 / g : 1;
+    auto cost = [&](int pl) {
+      double tot = 0;
+      int64_t cnt = 0;
+      const int64_t chunks = n / V;
+      for (int64_t t0 = 0; t0 < chunks && t0 < 64 * 64; t0 += 64)
+        for (int64_t v = 0; v < V; ++v)
+          for (int banks : {32, 64}) {
+            for (int64_t h = 0; h < 64; h += banks) {
+              int hist[64] = {0};
+              int worst = 0;
+              for (int64_t l = h; l < h + banks && t0 + l < chunks; ++l) {
+                const int64_t x = (int64_t)t->src[(t0 + l) * V + v] * g;
+                const int64_t a = pl ? x + (x >> pl) * 4 : x;
+                worst = std::max(worst, ++hist[(a / 4) % banks]);
+              }
+              tot += worst;
+              ++cnt;
+            }
+          }
+      return cnt ? tot / cnt : 0.0;
+    };
+    double best = cost(0) * 0.999;
+    for (int pl : {6, 7, 8, 9}) {  // 64..512 bytes between pads
+      const double c = cost(pl);
+      if (c < best) {
+        best = c;
+        t->pad_log2 = pl;
+      }
+    }
+  }
+  return t;
+}
+
+void fuse_tiles(std::vector<Op>& ops, const DeviceProfile& prof) {
+  if (prof.fuse_max_block <= 0) return;
+  std::vector<Op> out;
+  size_t i = 0;
+  while (i < ops.size()) {
+    size_t j = i + 1;
+    int64_t F = 0, g = 0;
+    if (fusable_tile(ops[i])) {
+      F = ops[i].R * ops[i].C * ops[i].U;
+      g = ops[i].U;
+      const uint64_t total = ops[i].bytes();
+      while (j < ops.size() && fusable_tile(ops[j]) && ops[j].base == ops[i].base &&
+             ops[j].bytes() == total) {
+        const int64_t Fj = ops[j].R * ops[j].C * ops[j].U;
+        const int64_t Fn = std::max(F, Fj);
+        // (the padded block, under 1/16 larger, must still fit LDS)
+        if ((F % Fj && Fj % F) || Fn > prof.fuse_max_block || Fn + Fn / 16 + 16 > prof.lds_max)
+          break;
+        F = Fn;
+        g = gcd64(g, ops[j].U);
+        ++j;
+      }
+    }
+    if (j - i < 2) {
+      out.push_back(ops[i]);
+      ++i;
+      continue;
+    }
+    Op b;
+    b.kind = OP_BLOCK;
+    b.stage = ops[i].stage;
+    b.base = ops[i].base;
+    b.B = (int64_t)(ops[i].bytes() / (uint64_t)F);
+    b.R = F / g;
+    b.C = 1;
+    b.U = g;
+    b.us = g;
+    b.blk = compose_tiles(ops, i, j, F, g);
+    out.push_back(b);
+    i = j;
+  }
+  ops.swap(out);
+}
+
+}  // namespace
+
+// `reserve`: bytes of the alpha budget kept back for the pass tables.
+static int make_plan_impl(const int64_t* dims_in, const int* perm1, int rank, size_t elem,
+                          int n_sub, double alpha, const DeviceProfile& prof, int reorder,
+                          int64_t reserve, Plan& plan) {
   plan = Plan();
   if (rank < 1 || rank > 64 || !dims_in || !perm1) return EITHOT_ERR_INVALID;
   if (elem == 0 || elem > (1u << 20)) return EITHOT_ERR_ELEM;
@@ -748,8 +865,8 @@ int make_plan(const int64_t* dims_in, const int* perm1, int rank, size_t elem, i
   std::vector<Stage> stages;
   if (!decompose(dims.data(), perm1, rank, reorder, stages)) return EITHOT_ERR_INVALID;
   plan.stages = stages;
-  const int64_t budget =
-      std::max<int64_t>((int64_t)(alpha * (double)plan.tensor_bytes), (int64_t)1);
+  const int64_t budget = std::max<int64_t>(
+      (int64_t)(alpha * (double)plan.tensor_bytes) - reserve, (int64_t)1);
   Lowerer L(prof, budget);
 
   // Sub-tensor partitioning along the largest dim (Nd_transpose.cpp:265-300):
@@ -796,27 +913,47 @@ int make_plan(const int64_t* dims_in, const int* perm1, int rank, size_t elem, i
       L.emit(st.S, st.Y, st.X, st.P * (int64_t)elem, 0, (int)s, 0, plan.ops);
     }
   }
-  int64_t fp_max = 0;
-  for (const Op& op : plan.ops) {
+  for (const Op& op : plan.ops)
     if (op.kind == 0) return EITHOT_ERR_ALPHA;
+  fuse_tiles(plan.ops, prof);
+  for (const Op& op : plan.ops) {
     plan.scratch_bytes = std::max(plan.scratch_bytes, staged_scratch(op, budget, prof));
-    if (op.staged) fp_max = std::max(fp_max, op.footprint_bytes());
     if (op.cyc) plan.table_bytes += op.cyc->bytes();
+    if (op.blk) plan.table_bytes += op.blk->bytes();
   }
-  // the cycle tables count as extra memory too: without cycle following if
-  // they would push the plan over its budget
-  if (plan.table_bytes > 0 && plan.scratch_bytes + plan.table_bytes > std::max(budget, fp_max) &&
-      prof.cycle_min_cell > 0) {
-    DeviceProfile p2 = prof;
-    p2.cycle_min_cell = 0;
-    return make_plan(dims_in, perm1, rank, elem, n_sub, alpha, p2, reorder, plan);
+  return EITHOT_OK;
+}
+
+int make_plan(const int64_t* dims_in, const int* perm1, int rank, size_t elem, int n_sub,
+              double alpha, const DeviceProfile& prof, int reorder, Plan& plan) {
+  int rc = make_plan_impl(dims_in, perm1, rank, elem, n_sub, alpha, prof, reorder, 0, plan);
+  if (rc || plan.table_bytes == 0) return rc;
+  // The pass tables count as extra memory too.  When they push the plan
+  // over its budget: plan again with their bytes kept back from the
+  // scratch budget; then without cycle following; then without fusion;
+  // keep whichever needs the least extra memory.
+  const int64_t budget = (int64_t)(alpha * (double)plan.tensor_bytes);
+  auto extra = [](const Plan& p) { return p.scratch_bytes + p.table_bytes; };
+  if (extra(plan) <= budget) return rc;
+  Plan best = plan;
+  DeviceProfile p2 = prof;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    int64_t reserve = 0;
+    if (attempt == 0) reserve = plan.table_bytes;
+    if (attempt == 1) p2.cycle_min_cell = 0;
+    if (attempt == 2) p2.fuse_max_block = 0;
+    Plan q;
+    if (make_plan_impl(dims_in, perm1, rank, elem, n_sub, alpha, p2, reorder, reserve, q)) continue;
+    if (extra(q) < extra(best)) best = q;
+    if (extra(best) <= budget) break;
   }
+  plan = best;
   return EITHOT_OK;
 }
 
 std::string Plan::describe() const {
   std::ostringstream os;
-  static const char* kn[] = {"?", "tile", "col", "row", "cycle"};
+  static const char* kn[] = {"?", "tile", "col", "row", "cycle", "block"};
   static const char* mn[] = {"-", "rot", "rotinv", "colsh", "colshinv", "rowg", "rowd"};
   os << "stages:";
   for (const Stage& s : stages) {

@@ -35,7 +35,7 @@ bool decompose(const int64_t* dims, const int* perm1, int rank, int reorder,
                std::vector<Stage>& out);
 
 // ---- device passes ---------------------------------------------------------
-enum OpKind : int { OP_TILE = 1, OP_COL = 2, OP_ROW = 3, OP_CYCLE = 4 };
+enum OpKind : int { OP_TILE = 1, OP_COL = 2, OP_ROW = 3, OP_CYCLE = 4, OP_BLOCK = 5 };
 
 // Index maps of the passes, all in gather form (dest <- source).
 enum MapKind : int {
@@ -73,6 +73,23 @@ struct CycleTable {
 // Builds the table (fixed points are skipped).  R*C must be below 2^32.
 std::shared_ptr<CycleTable> build_cycle_table(int64_t R, int64_t C, int64_t lmax);
 
+// Gather table of a fused block permutation (OP_BLOCK): consecutive
+// contiguous tile passes whose matrices nest inside blocks of `cells` cells
+// of `cell_bytes` bytes compose into one permutation of every block; output
+// cell o of a block takes input cell src[o].
+struct BlockTable {
+  int64_t cells = 0, cell_bytes = 0;
+  int pad_log2 = 0;  // > 0: LDS holds 4 pad bytes after every 2^pad_log2 bytes of a block
+  std::vector<uint32_t> src;
+  int64_t lds_bytes() const {
+    const int64_t F = cells * cell_bytes;
+    return pad_log2 ? F + (F >> pad_log2) * 4 + 4 : F;
+  }
+  int64_t bytes() const { return (int64_t)src.size() * (cells <= 65536 ? 2 : 4); }
+  mutable std::mutex dev_mu;
+  mutable std::map<int, std::shared_ptr<void>> dev;  // device copy (uint16 or uint32 indices)
+};
+
 struct Op {
   int kind = 0;
   int map = MAP_NONE;
@@ -87,6 +104,7 @@ struct Op {
   int64_t m = 1, n = 1, c = 1, a = 1, b = 1, k = 0, binv = 0;
 
   std::shared_ptr<const CycleTable> cyc;  // OP_CYCLE: segments shared by all B matrices
+  std::shared_ptr<const BlockTable> blk;  // OP_BLOCK: B blocks of R cells of U bytes
 
   int64_t footprint_units() const;   // units staged per footprint
   int64_t footprint_bytes() const { return footprint_units() * us; }
@@ -106,6 +124,7 @@ struct DeviceProfile {
   int64_t slice_min = 128;
   int64_t stage_chunk = 0;        // > 0: bytes of scratch one staged group may fill
   int64_t cycle_min_cell = 512;   // smallest cell moved by cycle following (0: off)
+  int64_t fuse_max_block = 64 * 1024;  // largest fused block permutation (0: no fusion)
   int reorder = 0;                // decomposer order (Reorder)
 };
 
@@ -117,6 +136,7 @@ struct Tuning {
   int64_t slice_min = 128;
   int64_t stage_chunk = 0;
   int64_t cycle_min_cell = 512;
+  int64_t fuse_max_block = 64 * 1024;
   int64_t threads = 256;
   int64_t threads_big = 512;
   int64_t xcd_remap = 1;

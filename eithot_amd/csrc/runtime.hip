@@ -373,6 +373,22 @@ int eithot_plan_describe(const int64_t* dims, const int* perm, int order, size_t
   return n;
 }
 
+int64_t eithot_plan_block_table(const int64_t* dims, const int* perm, int order,
+                                size_t elem_bytes, int n_sub, double alpha, int64_t lds_max,
+                                int op_index, uint32_t* src, int64_t max_cells) {
+  DeviceProfile prof;
+  if (lds_max > 0) prof.lds_max = lds_max;
+  tuning_apply(tuning_get(), prof);
+  Plan plan;
+  int rc = make_plan(dims, perm, order, elem_bytes, n_sub, alpha, prof, prof.reorder, plan);
+  if (rc) return fail(rc, "planning failed (%d)", rc);
+  if (op_index < 0 || op_index >= (int)plan.ops.size() || !plan.ops[op_index].blk)
+    return fail(EITHOT_ERR_INVALID, "pass %d is not a fused block permutation", op_index);
+  const BlockTable& t = *plan.ops[op_index].blk;
+  for (int64_t i = 0; i < t.cells && i < max_cells && src; ++i) src[i] = t.src[i];
+  return t.cells;
+}
+
 int eithot_set_tuning(const char* key, int64_t value) {
   int rc = tuning_set(key, value);
   if (rc) return fail(rc, "unknown tuning key or value out of range");

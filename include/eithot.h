@@ -107,7 +107,8 @@ int eithot_decompose(const int64_t* dims, const int* perm, int order, int reorde
 
 /* One device pass of a plan, for introspection and tests. */
 typedef struct {
-  int kind;      /* 1 tile, 2 column, 3 row, 4 tile by cycle following */
+  int kind;      /* 1 tile, 2 column, 3 row, 4 tile by cycle following,
+                    5 fused block permutation (B blocks of R cells of U bytes) */
   int map;       /* 0 none, 1 rot, 2 rotinv, 3 colsh, 4 colshinv, 5 rowg, 6 rowd */
   int staged;    /* 1 when executed through scratch */
   int stage;     /* decomposer stage, -1 for partition/join */
@@ -124,6 +125,13 @@ int eithot_plan_describe(const int64_t* dims, const int* perm, int order, size_t
                          int n_sub, double alpha, int64_t lds_max, eithot_op_desc* ops,
                          int max_ops, int64_t* scratch_bytes);
 
+/* Gather table of pass op_index when it is a fused block permutation:
+ * output cell o of every block takes input cell src[o].  Returns the cell
+ * count (copies at most max_cells) or a negative EITHOT_ERR_*. */
+int64_t eithot_plan_block_table(const int64_t* dims, const int* perm, int order,
+                                size_t elem_bytes, int n_sub, double alpha, int64_t lds_max,
+                                int op_index, uint32_t* src, int64_t max_cells);
+
 /* ---- tuning -------------------------------------------------------------------
  * No counterpart in the reference, whose equivalents are compile-time
  * constants (REORDER_TYPE, Nd_transpose.cpp:28; tier thresholds in col_op.cu
@@ -134,6 +142,8 @@ int eithot_plan_describe(const int64_t* dims, const int* perm, int order, size_t
  *   "slice_min"      smallest slice such slicing may choose (128)
  *   "stage_chunk"    > 0: bytes of scratch one staged group may fill (0)
  *   "cycle_min_cell" smallest cell (bytes) moved by cycle following (512; 0 = off)
+ *   "fuse_max_block" largest block of a fused tile permutation (65536; 0 = off)
+ *   "cycle_variant"  cycle-following kernel shape (0..7; 0 default)
  *   "threads", "threads_big"  workgroup sizes (256, 512; multiples of 64)
  *   "xcd_remap"      keep consecutive footprints on one XCD (1)
  * Returns EITHOT_ERR_INVALID for an unknown key or an out-of-range value. */

@@ -19,6 +19,13 @@ def simulate(buf, ops):
     i = 0
     while i < len(ops):
         o = ops[i]
+        if o["kind"] == 5:  # fused block permutation: out cell o <- in cell table[o]
+            U, n = o["U"], o["R"]
+            nb = o["B"] * n * U
+            blk = view[o["base"]:o["base"] + nb].reshape(o["B"], n, U)
+            view[o["base"]:o["base"] + nb] = blk[:, o["table"], :].reshape(-1)
+            i += 1
+            continue
         if o["kind"] in (1, 4):  # tile, or the same transpose by cycle following
             sub = view[o["base"]:]
             rc = O.transpose2d_direct(np.ascontiguousarray(sub), 0, 0, 0, 1)  # no-op warmup
@@ -31,7 +38,7 @@ def simulate(buf, ops):
         key = (o["base"], o["B"], o["R"], o["C"], o["U"])
         grp = [o]
         j = i + 1
-        while j < len(ops) and ops[j]["kind"] not in (1, 4) and (
+        while j < len(ops) and ops[j]["kind"] not in (1, 4, 5) and (
                 ops[j]["base"], ops[j]["B"], ops[j]["R"], ops[j]["C"], ops[j]["U"]) == key and \
                 len(grp) < 3 and (ops[j]["kind"], ops[j]["map"]) != (2, 1) and \
                 (ops[j]["kind"], ops[j]["map"]) != (2, 4):

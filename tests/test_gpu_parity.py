@@ -87,6 +87,29 @@ def test_cycle_following_passes(torch_cuda):
         E.set_tuning("cycle_variant", default)
 
 
+def test_fused_block_passes(torch_cuda):
+    # nested contiguous tile passes fused into one LDS block permutation
+    # (padded and unpadded LDS layouts, 1/2/4/8-byte cells), and the same
+    # tensors with fusion off
+    import eithot_amd as E
+    cases = [([12, 9, 21, 15, 17], [2, 5, 4, 1, 3], 2), ([48, 40, 8, 300], [3, 2, 1, 4], 4),
+             ([6, 5, 7, 9, 11], [2, 4, 3, 1, 5], 8), ([10, 13, 18, 12, 23], [5, 3, 4, 2, 1], 1),
+             ([2, 6, 20, 3], [1, 4, 3, 2], 8), ([13, 15, 10, 14, 10, 14], [4, 2, 3, 1, 5, 6], 1),
+             ([32, 48, 64, 5, 6, 8], [6, 4, 2, 5, 1, 3], 4)]
+    default = E.get_tuning("fuse_max_block")
+    try:
+        for fuse in (default, 0):
+            E.set_tuning("fuse_max_block", fuse)
+            for dims, perm, elem in cases:
+                ops, _ = E.plan_describe(dims, perm, elem, 1, 0.5)
+                if fuse:
+                    assert any(o["kind"] == 5 for o in ops), (dims, perm, ops)
+                got, want, _ = run_case(torch_cuda, dims, perm, elem, seed=9, alpha=0.5)
+                assert_bitexact(got, want, f"fuse {fuse} {dims} {perm} elem {elem}")
+    finally:
+        E.set_tuning("fuse_max_block", default)
+
+
 def test_random_high_order(torch_cuda):
     rng = np.random.default_rng(123)
     for _ in range(60):

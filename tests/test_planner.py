@@ -127,6 +127,34 @@ def test_narrow_slice_plans_execute_correctly(dims, perm1, elem, alpha, lds):
     assert np.array_equal(b, O.permute(a, dims, [p - 1 for p in perm1], elem))
 
 
+def test_fused_block_plans_execute_correctly():
+    # consecutive contiguous tile passes whose blocks nest become one block
+    # permutation; its gather table must compose to the oracle's permute
+    rng = np.random.default_rng(5)
+    fused = 0
+    for _ in range(300):
+        r = int(rng.integers(3, 7))
+        dims = [int(x) for x in rng.integers(2, 20, size=r)]
+        perm1 = [int(p) + 1 for p in rng.permutation(r)]
+        elem = int(rng.choice([1, 2, 4, 8]))
+        ops, scratch = E.plan_describe(dims, perm1, elem, 1, 0.5)
+        if not any(o["kind"] == 5 for o in ops):
+            continue
+        fused += 1
+        vol = int(np.prod(dims))
+        assert scratch <= 0.5 * vol * elem
+        a = O.fill(vol, elem, 3, 1)
+        b = a.copy()
+        simulate(b, ops)
+        assert np.array_equal(b, O.permute(a, dims, [p - 1 for p in perm1], elem)), (dims, perm1)
+    assert fused > 30
+    # the benchmark tensor: its last two tile passes fuse into one 60 KiB block pass
+    ops, _ = E.plan_describe([32, 48, 64, 40, 36, 8], [6, 4, 2, 5, 1, 3], 4, 1, 0.05)
+    blk = [o for o in ops if o["kind"] == 5]
+    assert len(blk) == 1 and blk[0]["R"] * blk[0]["U"] == 48 * 40 * 8 * 4
+    assert sorted(blk[0]["table"].tolist()) == list(range(48 * 40 * 8))
+
+
 def test_partition_plan_executes_correctly():
     for dims, perm1, nsub in (([152, 150], [2, 1], 4), ([6, 400, 3], [3, 2, 1], 8),
                               ([96, 40, 5], [2, 3, 1], 4)):
@@ -146,7 +174,7 @@ def test_config_plans():
     assert scratch == 0
     # config 2: one pass per reference stage, scratch within 5 % of 4.53 GB
     ops, scratch = E.plan_describe([32, 48, 64, 40, 36, 8], [6, 4, 2, 5, 1, 3], 4, 1, 0.05)
-    assert len(ops) <= 5 and all(o["kind"] in (1, 4) for o in ops)
+    assert len(ops) <= 5 and all(o["kind"] in (1, 4, 5) for o in ops)
     assert scratch <= 0.05 * 32 * 48 * 64 * 40 * 36 * 8 * 4
     # config 3: scratch within 5 % of 171.8 GB
     ops, scratch = E.plan_describe([2, 163840, 65536, 2], [1, 3, 2, 4], 4, 1, 0.05)
