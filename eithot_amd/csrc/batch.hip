@@ -94,9 +94,12 @@ struct eithot_batch {
     for (const LaunchStage& x : st) {
       if (batch_launch_enqueue(x.persistent, s) != hipSuccess)
         return fail(EITHOT_ERR_DEVICE, "batched launch failed");
-      for (const auto& so : x.side)
-        if (launch_op(*so.first, so.second, scratch, scratch_bytes, c, s) != hipSuccess)
-          return fail(EITHOT_ERR_DEVICE, "batched side pass failed");
+      for (const auto& so : x.side) {
+        const hipError_t e = launch_op(*so.first, so.second, scratch, scratch_bytes, c, s);
+        if (e != hipSuccess)
+          return fail(EITHOT_ERR_DEVICE, "batched side pass (kind %d) failed: %s", so.first->kind,
+                      hipGetErrorString(e));
+      }
     }
     return EITHOT_OK;
   }

@@ -1370,7 +1370,26 @@ struct BlkArgs {
   uint32_t n, wpc, K, vec16, idx16, chunks;  // chunks: n / V
   uint32_t pad, Fl;                           // pad: log2 of the pad interval (0: none); Fl: LDS bytes per block
   FD32 fnw, fwpc, fch, fF16, fFw;             // n * wpc, wpc, chunks, F / 16, F / w
+  uint32_t ndig, stride[8];                   // digit form of the table (ndig = 0: table)
+  FD32 frad[8];
 };
+
+// source cell of output cell o: digit form, or the gather table
+__device__ __forceinline__ uint32_t blk_src(const BlkArgs& a, uint32_t o) {
+  if (a.ndig) {
+    uint32_t q = o, v = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (k >= (int)a.ndig) break;
+      const uint32_t nq = a.frad[k].div(q);
+      v += (q - nq * a.frad[k].d) * a.stride[k];
+      q = nq;
+    }
+    return v;
+  }
+  return a.idx16 ? (uint32_t) reinterpret_cast<const uint16_t*>(a.table)[o]
+                 : reinterpret_cast<const uint32_t*>(a.table)[o];
+}
 
 // LDS byte offset of byte x of a block (4 pad bytes after every 2^pad bytes)
 __device__ __forceinline__ uint32_t lds_off(uint32_t x, uint32_t pad) {
@@ -1378,7 +1397,7 @@ __device__ __forceinline__ uint32_t lds_off(uint32_t x, uint32_t pad) {
 }
 
 template <typename Word, int V>
-__global__ __launch_bounds__(256) void k_block(BlkArgs a, int remap) {
+__global__ __launch_bounds__(512) void k_block(BlkArgs a, int remap) {
   extern __shared__ __align__(16) unsigned char lds[];
   constexpr uint32_t w = sizeof(Word);
   const uint64_t f0 = (uint64_t)xcd_logical(blockIdx.x, gridDim.x, remap) * a.K;
@@ -1414,8 +1433,6 @@ __global__ __launch_bounds__(256) void k_block(BlkArgs a, int remap) {
     }
   }
   __syncthreads();
-  const uint16_t* T16 = reinterpret_cast<const uint16_t*>(a.table);
-  const uint32_t* T32 = reinterpret_cast<const uint32_t*>(a.table);
   char* out = a.data + a.base + f0 * a.F;
   if (V > 1) {
     // one word per cell, V cells per 16-byte store
@@ -1424,10 +1441,18 @@ __global__ __launch_bounds__(256) void k_block(BlkArgs a, int remap) {
       const uint32_t k = a.fch.div(t);
       const uint32_t o0 = (t - k * a.chunks) * V;
       Word tmp[V];
+      if (a.ndig && a.frad[0].d % V == 0) {
+        // the V cells differ in the fastest output digit only
+        const uint32_t s0 = blk_src(a, o0);
 #pragma unroll
-      for (int v = 0; v < V; ++v) {
-        const uint32_t src = a.idx16 ? (uint32_t)T16[o0 + v] : T32[o0 + v];
-        tmp[v] = *reinterpret_cast<const Word*>(lds + k * a.Fl + lds_off(src * w, a.pad));
+        for (int v = 0; v < V; ++v)
+          tmp[v] = *reinterpret_cast<const Word*>(
+              lds + k * a.Fl + lds_off((s0 + v * a.stride[0]) * w, a.pad));
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+          tmp[v] = *reinterpret_cast<const Word*>(lds + k * a.Fl +
+                                                  lds_off(blk_src(a, o0 + v) * w, a.pad));
       }
       *reinterpret_cast<Vec<V * sizeof(Word)>*>(out + k * a.F + (uint64_t)o0 * w) =
           *reinterpret_cast<const Vec<V * sizeof(Word)>*>(tmp);
@@ -1440,7 +1465,7 @@ __global__ __launch_bounds__(256) void k_block(BlkArgs a, int remap) {
       const uint32_t r = t - k * per;
       const uint32_t o = a.wpc == 1 ? r : a.fwpc.div(r);
       const uint32_t x = r - o * a.wpc;
-      const uint32_t src = a.idx16 ? (uint32_t)T16[o] : T32[o];
+      const uint32_t src = blk_src(a, o);
       reinterpret_cast<Word*>(out)[(uint64_t)k * per + r] = *reinterpret_cast<const Word*>(
           lds + k * a.Fl + lds_off((src * a.wpc + x) * w, a.pad));
     }
@@ -1487,9 +1512,9 @@ hipError_t launch_block(const Op& op, char* data, const LaunchCfg& cfg, hipStrea
   const BlockTable& t = *op.blk;
   const int64_t F = op.R * op.U;
   if (F > 160 * 1024 || op.R >= (1ll << 31) || op.B <= 0) return hipErrorInvalidValue;
-  hipError_t e;
-  const void* d = block_table_device(t, &e);
-  if (!d) return e;
+  hipError_t e = hipSuccess;
+  const void* d = nullptr;
+  if (!t.ndig && !(d = block_table_device(t, &e))) return e;
   const uintptr_t base = (uintptr_t)(data + op.base);
   int64_t w = 16;
   while (w > 1 && ((op.U % w) || (base % w) || (F % w))) w /= 2;
@@ -1520,6 +1545,11 @@ hipError_t launch_block(const Op& op, char* data, const LaunchCfg& cfg, hipStrea
   a.fch.init(a.chunks);
   a.fF16.init((uint32_t)std::max<int64_t>(F / 16, 1));
   a.fFw.init((uint32_t)(F / w));
+  a.ndig = (uint32_t)t.ndig;
+  for (int k = 0; k < 8; ++k) {
+    a.frad[k].init(k < t.ndig ? t.radix[k] : 1);
+    a.stride[k] = k < t.ndig ? t.stride[k] : 0;
+  }
   const int64_t groups = (op.B + K - 1) / K;
   if (groups >= (1ll << 31)) return hipErrorInvalidValue;
   const int64_t lds = K * (int64_t)a.Fl;
@@ -1537,7 +1567,7 @@ hipError_t launch_block(const Op& op, char* data, const LaunchCfg& cfg, hipStrea
 hipError_t prepare_op(const Op& op) {
   hipError_t e = hipSuccess;
   if (op.kind == OP_CYCLE && op.cyc && !op.cyc->segs.empty()) cycle_table_device(*op.cyc, &e);
-  if (op.kind == OP_BLOCK && op.blk) block_table_device(*op.blk, &e);
+  if (op.kind == OP_BLOCK && op.blk && !op.blk->ndig) block_table_device(*op.blk, &e);
   return e;
 }
 

@@ -675,6 +675,7 @@ static int64_t* tuning_field(Tuning& t, const char* key) {
   if (!strcmp(key, "stage_chunk")) return &t.stage_chunk;
   if (!strcmp(key, "cycle_min_cell")) return &t.cycle_min_cell;
   if (!strcmp(key, "fuse_max_block")) return &t.fuse_max_block;
+  if (!strcmp(key, "block_digits")) return &t.block_digits;
   if (!strcmp(key, "threads")) return &t.threads;
   if (!strcmp(key, "threads_big")) return &t.threads_big;
   if (!strcmp(key, "xcd_remap")) return &t.xcd_remap;
@@ -689,7 +690,7 @@ int tuning_set(const char* key, int64_t value) {
   int64_t* f = tuning_field(t, key);
   if (!f || value < 0) return EITHOT_ERR_INVALID;
   const bool threads = f == &t.threads || f == &t.threads_big;
-  if (threads && (value < 64 || value > 1024 || value % 64)) return EITHOT_ERR_INVALID;
+  if (threads && (value < 64 || value > 512 || value % 64)) return EITHOT_ERR_INVALID;
   if (f == &t.reorder && value > 2) return EITHOT_ERR_INVALID;
   if (f == &t.cycle_variant && value > 7) return EITHOT_ERR_INVALID;
   if (f == &t.lds_target && value < 1024) return EITHOT_ERR_INVALID;
@@ -715,6 +716,7 @@ void tuning_apply(const Tuning& t, DeviceProfile& prof) {
   prof.stage_chunk = t.stage_chunk;
   prof.cycle_min_cell = t.cycle_min_cell;
   prof.fuse_max_block = std::min<int64_t>(t.fuse_max_block, prof.lds_max);
+  prof.block_digits = t.block_digits;
   prof.reorder = (int)t.reorder;
 }
 
@@ -744,7 +746,7 @@ namespace {
 bool fusable_tile(const Op& o) { return o.kind == OP_TILE && !o.staged && o.us == o.U; }
 
 std::shared_ptr<BlockTable> compose_tiles(const std::vector<Op>& ops, size_t i0, size_t i1,
-                                          int64_t F, int64_t g) {
+                                          int64_t F, int64_t g, bool digits) {
   auto t = std::make_shared<BlockTable>();
   const int64_t n = F / g;
   t->cells = n;
@@ -762,6 +764,40 @@ std::shared_ptr<BlockTable> compose_tiles(const std::vector<Op>& ops, size_t i0,
     cur.swap(nxt);
   }
   t->src = std::move(cur);
+  // digit form: greedy radices from the table, then checked on every cell
+  {
+    const std::vector<uint32_t>& T = t->src;
+    int64_t step = 1;
+    int nd = 0;
+    bool ok = T[0] == 0;
+    while (ok && step < n) {
+      if (nd == 8) {
+        ok = false;
+        break;
+      }
+      const int64_t st = T[step];
+      int64_t r = 1;
+      while (step * (r + 1) <= n && (int64_t)T[step * r] == r * st) ++r;
+      while (r > 1 && n % (step * r)) --r;
+      if (r < 2) {
+        ok = false;
+        break;
+      }
+      t->radix[nd] = (uint32_t)r;
+      t->stride[nd] = (uint32_t)st;
+      ++nd;
+      step *= r;
+    }
+    for (int64_t o = 0; ok && o < n; ++o) {
+      int64_t q = o, v = 0;
+      for (int k = 0; k < nd; ++k) {
+        v += (q % t->radix[k]) * (int64_t)t->stride[k];
+        q /= t->radix[k];
+      }
+      ok = v == (int64_t)T[o];
+    }
+    t->ndig = ok && digits ? nd : 0;
+  }
   // LDS padding: the gather reads V = 16/g cells per thread (one word
   // each) across 64 lanes; pick the pad interval with the fewest bank
   // conflicts (64 banks of 4 bytes, and 32-lane halves on 32 banks)
@@ -837,7 +873,7 @@ void fuse_tiles(std::vector<Op>& ops, const DeviceProfile& prof) {
     b.C = 1;
     b.U = g;
     b.us = g;
-    b.blk = compose_tiles(ops, i, j, F, g);
+    b.blk = compose_tiles(ops, i, j, F, g, prof.block_digits != 0);
     out.push_back(b);
     i = j;
   }

@@ -81,11 +81,17 @@ struct BlockTable {
   int64_t cells = 0, cell_bytes = 0;
   int pad_log2 = 0;  // > 0: LDS holds 4 pad bytes after every 2^pad_log2 bytes of a block
   std::vector<uint32_t> src;
+  // When the permutation is a digit permutation (it is for nested
+  // transposes): with o's mixed-radix digits d_k (radix[k], fastest first),
+  // src[o] = sum d_k * stride[k].  ndig = 0: use the table.
+  int ndig = 0;
+  uint32_t radix[8] = {0}, stride[8] = {0};
   int64_t lds_bytes() const {
     const int64_t F = cells * cell_bytes;
     return pad_log2 ? F + (F >> pad_log2) * 4 + 4 : F;
   }
-  int64_t bytes() const { return (int64_t)src.size() * (cells <= 65536 ? 2 : 4); }
+  // device bytes: the gather table, unless the digit form replaces it
+  int64_t bytes() const { return ndig ? 0 : (int64_t)src.size() * (cells <= 65536 ? 2 : 4); }
   mutable std::mutex dev_mu;
   mutable std::map<int, std::shared_ptr<void>> dev;  // device copy (uint16 or uint32 indices)
 };
@@ -125,6 +131,7 @@ struct DeviceProfile {
   int64_t stage_chunk = 0;        // > 0: bytes of scratch one staged group may fill
   int64_t cycle_min_cell = 512;   // smallest cell moved by cycle following (0: off)
   int64_t fuse_max_block = 64 * 1024;  // largest fused block permutation (0: no fusion)
+  int64_t block_digits = 1;       // fused blocks index cells arithmetically when they can
   int reorder = 0;                // decomposer order (Reorder)
 };
 
@@ -137,6 +144,7 @@ struct Tuning {
   int64_t stage_chunk = 0;
   int64_t cycle_min_cell = 512;
   int64_t fuse_max_block = 64 * 1024;
+  int64_t block_digits = 1;
   int64_t threads = 256;
   int64_t threads_big = 512;
   int64_t xcd_remap = 1;

@@ -143,8 +143,10 @@ int64_t eithot_plan_block_table(const int64_t* dims, const int* perm, int order,
  *   "stage_chunk"    > 0: bytes of scratch one staged group may fill (0)
  *   "cycle_min_cell" smallest cell (bytes) moved by cycle following (512; 0 = off)
  *   "fuse_max_block" largest block of a fused tile permutation (65536; 0 = off)
+ *   "block_digits"   fused blocks compute source cells from digits (1) or read
+ *                    a gather table (0)
  *   "cycle_variant"  cycle-following kernel shape (0..7; 0 default)
- *   "threads", "threads_big"  workgroup sizes (256, 512; multiples of 64)
+ *   "threads", "threads_big"  workgroup sizes (256, 512; multiples of 64, at most 512)
  *   "xcd_remap"      keep consecutive footprints on one XCD (1)
  * Returns EITHOT_ERR_INVALID for an unknown key or an out-of-range value. */
 int eithot_set_tuning(const char* key, int64_t value);

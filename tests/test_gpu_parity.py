@@ -98,16 +98,18 @@ def test_fused_block_passes(torch_cuda):
              ([32, 48, 64, 5, 6, 8], [6, 4, 2, 5, 1, 3], 4)]
     default = E.get_tuning("fuse_max_block")
     try:
-        for fuse in (default, 0):
+        for fuse, digits in ((default, 1), (default, 0), (0, 1)):
             E.set_tuning("fuse_max_block", fuse)
+            E.set_tuning("block_digits", digits)
             for dims, perm, elem in cases:
                 ops, _ = E.plan_describe(dims, perm, elem, 1, 0.5)
                 if fuse:
                     assert any(o["kind"] == 5 for o in ops), (dims, perm, ops)
                 got, want, _ = run_case(torch_cuda, dims, perm, elem, seed=9, alpha=0.5)
-                assert_bitexact(got, want, f"fuse {fuse} {dims} {perm} elem {elem}")
+                assert_bitexact(got, want, f"fuse {fuse} digits {digits} {dims} {perm} elem {elem}")
     finally:
         E.set_tuning("fuse_max_block", default)
+        E.set_tuning("block_digits", 1)
 
 
 def test_random_high_order(torch_cuda):
