@@ -125,6 +125,12 @@ def plan_describe(dims, perm1, elem_bytes, n_sub=1, alpha=0.05, lds_max=0):
                 i64(dims), i32(perm1), len(dims), int(elem_bytes), int(n_sub), float(alpha),
                 int(lds_max), i, tab.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), tab.size))
             d["table"] = tab
+        if d["kind"] == 4:  # cycle following: attach its cell maps (bc, R, C, m)
+            mp = (ctypes.c_int64 * 16)()
+            k = check(lib().eithot_plan_cycle_maps(
+                i64(dims), i32(perm1), len(dims), int(elem_bytes), int(n_sub), float(alpha),
+                int(lds_max), i, mp, 4))
+            d["maps"] = [tuple(mp[4 * q:4 * q + 4]) for q in range(k)]
         out.append(d)
     return out, scratch.value
 

@@ -1075,17 +1075,35 @@ namespace {
 // cell saved in a boundary slot by k_cycle_save (cut cycles) or the cell the
 // wave read from its first position before writing it (whole cycles).
 // ---------------------------------------------------------------------------
+struct CycMapDev {
+  uint32_t bc, R, C, m;
+  FD32 fbc, fm, fR;
+};
 struct CycArgs {
   char* data;
   uint64_t base, RCU, U;
   const CycleSeg* segs;
   const uint32_t* slots;
   char* boundary;
-  uint32_t nseg, nslots, R, C, chunk_bytes;
+  uint32_t nseg, nslots, nmaps, chunk_bytes;
   uint64_t B, items, chunks;
-  FD32 fR;
+  CycMapDev maps[4];
   FD64 fchunks, fnseg;
 };
+
+// source position of cell position p: the pass's maps composed in gather order
+__device__ __forceinline__ uint32_t cyc_src(const CycArgs& a, uint32_t p) {
+#pragma unroll
+  for (int k = 3; k >= 0; --k) {
+    if (k >= (int)a.nmaps) continue;
+    const CycMapDev& M = a.maps[k];
+    const uint32_t b = M.fbc.div(p), q = p - b * M.bc;
+    const uint32_t c = M.fm.div(q), x = q - c * M.m;
+    const uint32_t j = M.fR.div(c), i = c - j * M.R;
+    p = b * M.bc + (i * M.C + j) * M.m + x;
+  }
+  return p;
+}
 
 template <typename Word>
 __global__ __launch_bounds__(256) void k_cycle_save(CycArgs a) {
@@ -1162,11 +1180,7 @@ __global__ __launch_bounds__(256) void k_cycle(CycArgs a) {
     uint32_t q[D];
     uint32_t pp = p;
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-      const uint32_t j = a.fR.div(pp);
-      pp = (pp - j * a.R) * a.C + j;  // src(pp): (i, j) -> i*C + j
-      q[k] = pp;
-    }
+    for (int k = 0; k < D; ++k) q[k] = pp = cyc_src(a, pp);
 #pragma unroll
     for (int k = 0; k < D; ++k)
       if (t + k < sg.len)
@@ -1212,10 +1226,7 @@ __global__ __launch_bounds__(256) void k_cycle_pipe(CycArgs a) {
 #pragma unroll
   for (int v = 0; v < V; ++v)
     if (on[v]) temp[v] = *reinterpret_cast<const Word*>(tsrc + v * 64 * w);
-  auto next = [&](uint32_t pp) {
-    const uint32_t j = a.fR.div(pp);
-    return (pp - j * a.R) * a.C + j;
-  };
+  auto next = [&](uint32_t pp) { return cyc_src(a, pp); };
   Word cur[D][V], nxt[D][V];
   uint32_t qc[D], qn[D];
   uint32_t p = sg.start;
@@ -1338,13 +1349,22 @@ hipError_t launch_cycle(const Op& op, char* data, char* scratch, int64_t scratch
   a.boundary = scratch;
   a.nseg = (uint32_t)t.segs.size();
   a.nslots = (uint32_t)t.slots.size();
-  a.R = (uint32_t)op.R;
-  a.C = (uint32_t)op.C;
+  if (t.maps.empty() || t.maps.size() > 4) return hipErrorInvalidValue;
+  a.nmaps = (uint32_t)t.maps.size();
+  for (size_t k = 0; k < 4; ++k) {
+    const CycleMap M = k < t.maps.size() ? t.maps[k] : CycleMap{1, 1, 1, 1};
+    a.maps[k].bc = (uint32_t)M.bc;
+    a.maps[k].R = (uint32_t)M.R;
+    a.maps[k].C = (uint32_t)M.C;
+    a.maps[k].m = (uint32_t)M.m;
+    a.maps[k].fbc.init((uint32_t)M.bc);
+    a.maps[k].fm.init((uint32_t)M.m);
+    a.maps[k].fR.init((uint32_t)M.R);
+  }
   a.chunk_bytes = (uint32_t)(64 * w * cycle_vwords(variant));
   a.B = op.B;
   a.chunks = (op.U + a.chunk_bytes - 1) / a.chunk_bytes;
   a.items = a.B * a.nseg * a.chunks;
-  a.fR.init(a.R);
   a.fchunks.init(a.chunks);
   a.fnseg.init(a.nseg);
   if ((a.items + 3) / 4 >= (1ull << 31)) return hipErrorInvalidValue;

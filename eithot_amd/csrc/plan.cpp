@@ -212,16 +212,24 @@ int64_t Op::num_footprints() const {
   return 0;
 }
 
-std::shared_ptr<CycleTable> build_cycle_table(int64_t R, int64_t C, int64_t lmax) {
+std::shared_ptr<CycleTable> build_cycle_table_maps(const std::vector<CycleMap>& maps, int64_t N,
+                                                   int64_t lmax) {
   auto t = std::make_shared<CycleTable>();
-  t->R = R;
-  t->C = C;
+  t->maps = maps;
   t->lmax = lmax;
-  const uint64_t N = (uint64_t)R * (uint64_t)C;
-  if (N >= (1ull << 32) || lmax < 1) return t;
-  auto src = [&](uint64_t p) { return (p % (uint64_t)R) * (uint64_t)C + p / (uint64_t)R; };
-  std::vector<uint8_t> seen(N, 0);
-  for (uint64_t p0 = 0; p0 < N; ++p0) {
+  if (N <= 0 || (uint64_t)N >= (1ull << 32) || lmax < 1 || maps.empty()) return t;
+  auto src = [&](uint64_t p) {
+    for (size_t k = maps.size(); k-- > 0;) {
+      const CycleMap& M = maps[k];
+      const uint64_t b = p / (uint64_t)M.bc, q = p - b * (uint64_t)M.bc;
+      const uint64_t c = q / (uint64_t)M.m, x = q - c * (uint64_t)M.m;
+      const uint64_t j = c / (uint64_t)M.R, i = c - j * (uint64_t)M.R;
+      p = b * (uint64_t)M.bc + (i * (uint64_t)M.C + j) * (uint64_t)M.m + x;
+    }
+    return p;
+  };
+  std::vector<uint8_t> seen((size_t)N, 0);
+  for (uint64_t p0 = 0; p0 < (uint64_t)N; ++p0) {
     if (seen[p0]) continue;
     uint64_t k = 0, q = p0;
     do {
@@ -238,7 +246,6 @@ std::shared_ptr<CycleTable> build_cycle_table(int64_t R, int64_t C, int64_t lmax
     // its last cell from the slot of segment i+1 (cyclically)
     const uint64_t n = (k + (uint64_t)lmax - 1) / (uint64_t)lmax;
     const uint32_t first_slot = (uint32_t)t->slots.size();
-    const size_t first_seg = t->segs.size();
     q = p0;
     for (uint64_t i = 0; i < n; ++i) {
       const uint64_t len = k / n + (i < k % n ? 1 : 0);
@@ -247,8 +254,14 @@ std::shared_ptr<CycleTable> build_cycle_table(int64_t R, int64_t C, int64_t lmax
                          (int32_t)(first_slot + (uint32_t)((i + 1) % n)), 0});
       for (uint64_t s = 0; s < len; ++s) q = src(q);
     }
-    (void)first_seg;
   }
+  return t;
+}
+
+std::shared_ptr<CycleTable> build_cycle_table(int64_t R, int64_t C, int64_t lmax) {
+  auto t = build_cycle_table_maps({CycleMap{R * C, R, C, 1}}, R * C, lmax);
+  t->R = R;
+  t->C = C;
   return t;
 }
 
@@ -411,6 +424,26 @@ std::vector<int64_t> divisors(int64_t x) {
   return d;
 }
 
+// Segment length of a cycle pass over B regions of N cells of U bytes:
+// short enough for about 32 Ki waves (one per segment and 1 KiB of cell),
+// long enough that the saved boundary cells and the table fit the budget.
+std::shared_ptr<CycleTable> choose_cycle_table(const std::vector<CycleMap>& maps, int64_t N,
+                                               int64_t B, int64_t U, int64_t budget) {
+  const int64_t chunk = 64 * pow2_factor(U, 16);
+  const int64_t chunks = (U + chunk - 1) / chunk;
+  int64_t lmax = 256;
+  std::shared_ptr<CycleTable> t = build_cycle_table_maps(maps, N, lmax);
+  while (lmax > 16 && B * (int64_t)t->segs.size() * chunks < 32768) {
+    lmax /= 2;
+    t = build_cycle_table_maps(maps, N, lmax);
+  }
+  while (!t->slots.empty() && B * (int64_t)t->slots.size() * U + t->bytes() > budget) {
+    lmax = lmax >= N ? N : lmax * 2;
+    t = build_cycle_table_maps(maps, N, lmax);
+  }
+  return t;
+}
+
 enum How {
   HOW_NONE = 0,
   HOW_TILE = 1,
@@ -520,22 +553,10 @@ struct Lowerer {
     return ch;
   }
 
-  // Segment length: short enough for about 32 Ki waves (one per segment and
-  // 1 KiB of cell), long enough that the saved boundary cells fit the budget.
   std::shared_ptr<const CycleTable> cycle_table_for(int64_t B, int64_t R, int64_t C, int64_t U) {
-    const int64_t chunk = 64 * pow2_factor(U, 16);
-    const int64_t chunks = (U + chunk - 1) / chunk;
-    int64_t lmax = 256;
-    std::shared_ptr<CycleTable> t = build_cycle_table(R, C, lmax);
-    while (lmax > 16 && B * (int64_t)t->segs.size() * chunks < 32768) {
-      lmax /= 2;
-      t = build_cycle_table(R, C, lmax);
-    }
-    const int64_t N = R * C;
-    while (!t->slots.empty() && B * (int64_t)t->slots.size() * U + t->bytes() > scratch_budget) {
-      lmax = lmax >= N ? N : lmax * 2;
-      t = build_cycle_table(R, C, lmax);
-    }
+    auto t = choose_cycle_table({CycleMap{R * C, R, C, 1}}, R * C, B, U, scratch_budget);
+    t->R = R;
+    t->C = C;
     return t;
   }
 
@@ -836,6 +857,58 @@ std::shared_ptr<BlockTable> compose_tiles(const std::vector<Op>& ops, size_t i0,
   return t;
 }
 
+// Consecutive passes that permute whole cells of at least cycle_min_cell
+// bytes over the same bytes (cycle passes and whole-unit tiles) compose into
+// one cycle pass over cells of their common size, when at least one of them
+// is a cycle pass.
+void fuse_cycles(std::vector<Op>& ops, const DeviceProfile& prof, int64_t budget) {
+  if (prof.cycle_min_cell <= 0) return;
+  auto cellperm = [&](const Op& o) {
+    return (o.kind == OP_CYCLE && o.cyc && o.cyc->maps.size() == 1) ||
+           (o.kind == OP_TILE && !o.staged && o.us == o.U);
+  };
+  std::vector<Op> out;
+  size_t i = 0;
+  while (i < ops.size()) {
+    size_t j = i;
+    int64_t g = 0;
+    int cycles = 0;
+    while (j < ops.size() && j - i < 4 && cellperm(ops[j]) && ops[j].base == ops[i].base &&
+           ops[j].bytes() == ops[i].bytes()) {
+      const int64_t g2 = gcd64(g, ops[j].U);
+      if (g2 < prof.cycle_min_cell || ops[j].bytes() / (uint64_t)g2 > kCycleMaxCells) break;
+      g = g2;
+      cycles += ops[j].kind == OP_CYCLE;
+      ++j;
+    }
+    if (j - i < 2 || !cycles) {
+      out.push_back(ops[i]);
+      ++i;
+      continue;
+    }
+    std::vector<CycleMap> maps;
+    for (size_t q = i; q < j; ++q) {
+      const Op& o = ops[q];
+      const int64_t m = o.U / g;
+      maps.push_back(CycleMap{o.R * o.C * m, o.R, o.C, m});
+    }
+    const int64_t N = (int64_t)(ops[i].bytes() / (uint64_t)g);
+    Op c;
+    c.kind = OP_CYCLE;
+    c.stage = ops[i].stage;
+    c.base = ops[i].base;
+    c.B = 1;
+    c.R = N;
+    c.C = 1;
+    c.U = g;
+    c.us = g;
+    c.cyc = choose_cycle_table(maps, N, 1, g, budget);
+    out.push_back(c);
+    i = j;
+  }
+  ops.swap(out);
+}
+
 void fuse_tiles(std::vector<Op>& ops, const DeviceProfile& prof) {
   if (prof.fuse_max_block <= 0) return;
   std::vector<Op> out;
@@ -951,6 +1024,7 @@ static int make_plan_impl(const int64_t* dims_in, const int* perm1, int rank, si
   }
   for (const Op& op : plan.ops)
     if (op.kind == 0) return EITHOT_ERR_ALPHA;
+  fuse_cycles(plan.ops, prof, budget);
   fuse_tiles(plan.ops, prof);
   for (const Op& op : plan.ops) {
     plan.scratch_bytes = std::max(plan.scratch_bytes, staged_scratch(op, budget, prof));

@@ -61,8 +61,17 @@ struct CycleSeg {
                      // segment is a whole cycle and takes old(start)
   uint32_t pad;
 };
+// One cell map of a cycle pass: cells in blocks of bc cells; a block is an
+// R x C matrix of m-cell units, transposed: cell q = (j*R + i)*m + x of a
+// block takes cell (i*C + j)*m + x.
+struct CycleMap {
+  int64_t bc, R, C, m;
+};
 struct CycleTable {
   int64_t R = 0, C = 0, lmax = 0;
+  // maps applied in pass order; the pass's source of position p is
+  // src_1(src_2(...src_k(p))) (gather composition)
+  std::vector<CycleMap> maps;
   std::vector<CycleSeg> segs;
   std::vector<uint32_t> slots;  // slot -> position whose old cell it holds
   int64_t bytes() const { return (int64_t)(segs.size() * sizeof(CycleSeg) + slots.size() * 4); }
@@ -72,6 +81,9 @@ struct CycleTable {
 };
 // Builds the table (fixed points are skipped).  R*C must be below 2^32.
 std::shared_ptr<CycleTable> build_cycle_table(int64_t R, int64_t C, int64_t lmax);
+// Same over the composition of several cell maps on N cells (N < 2^32).
+std::shared_ptr<CycleTable> build_cycle_table_maps(const std::vector<CycleMap>& maps, int64_t N,
+                                                   int64_t lmax);
 
 // Gather table of a fused block permutation (OP_BLOCK): consecutive
 // contiguous tile passes whose matrices nest inside blocks of `cells` cells

@@ -389,6 +389,27 @@ int64_t eithot_plan_block_table(const int64_t* dims, const int* perm, int order,
   return t.cells;
 }
 
+int eithot_plan_cycle_maps(const int64_t* dims, const int* perm, int order, size_t elem_bytes,
+                           int n_sub, double alpha, int64_t lds_max, int op_index,
+                           int64_t* maps, int max_maps) {
+  DeviceProfile prof;
+  if (lds_max > 0) prof.lds_max = lds_max;
+  tuning_apply(tuning_get(), prof);
+  Plan plan;
+  int rc = make_plan(dims, perm, order, elem_bytes, n_sub, alpha, prof, prof.reorder, plan);
+  if (rc) return fail(rc, "planning failed (%d)", rc);
+  if (op_index < 0 || op_index >= (int)plan.ops.size() || !plan.ops[op_index].cyc)
+    return fail(EITHOT_ERR_INVALID, "pass %d is not a cycle-following pass", op_index);
+  const std::vector<CycleMap>& mv = plan.ops[op_index].cyc->maps;
+  for (int k = 0; k < (int)mv.size() && k < max_maps && maps; ++k) {
+    maps[4 * k] = mv[k].bc;
+    maps[4 * k + 1] = mv[k].R;
+    maps[4 * k + 2] = mv[k].C;
+    maps[4 * k + 3] = mv[k].m;
+  }
+  return (int)mv.size();
+}
+
 int eithot_set_tuning(const char* key, int64_t value) {
   int rc = tuning_set(key, value);
   if (rc) return fail(rc, "unknown tuning key or value out of range");

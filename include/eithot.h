@@ -132,6 +132,14 @@ int64_t eithot_plan_block_table(const int64_t* dims, const int* perm, int order,
                                 size_t elem_bytes, int n_sub, double alpha, int64_t lds_max,
                                 int op_index, uint32_t* src, int64_t max_cells);
 
+/* Cell maps of pass op_index when it is a cycle-following pass, in pass
+ * order, 4 values each (bc, R, C, m): blocks of bc cells of U bytes, each an
+ * R x C matrix of m-cell units, transposed.  Returns the map count (copies
+ * at most max_maps) or a negative EITHOT_ERR_*. */
+int eithot_plan_cycle_maps(const int64_t* dims, const int* perm, int order, size_t elem_bytes,
+                           int n_sub, double alpha, int64_t lds_max, int op_index,
+                           int64_t* maps, int max_maps);
+
 /* ---- tuning -------------------------------------------------------------------
  * No counterpart in the reference, whose equivalents are compile-time
  * constants (REORDER_TYPE, Nd_transpose.cpp:28; tier thresholds in col_op.cu

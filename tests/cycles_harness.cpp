@@ -32,7 +32,16 @@ struct Ctx {
   uint64_t R, C, U, RCU;
   std::vector<uint8_t> boundary;
   uint8_t* cell(uint64_t b, uint32_t p, uint64_t off) { return buf + b * RCU + (uint64_t)p * U + off; }
-  uint32_t src(uint32_t p) const { return (uint32_t)((p % R) * C + p / R); }
+  uint32_t src(uint32_t p) const {
+    uint64_t q = p;
+    for (size_t k = tab->maps.size(); k-- > 0;) {
+      const eithot::CycleMap& M = tab->maps[k];
+      const uint64_t b = q / M.bc, r = q - b * M.bc, c = r / M.m, x = r - c * M.m;
+      const uint64_t j = c / M.R, i = c - j * M.R;
+      q = b * M.bc + (i * M.C + j) * M.m + x;
+    }
+    return (uint32_t)q;
+  }
 
   void start(Item& it) {
     const CycleSeg& sg = tab->segs[it.s];
@@ -74,13 +83,37 @@ struct Ctx {
 
 }  // namespace
 
+namespace {
+int64_t run_table(std::shared_ptr<CycleTable> tab, int64_t B, int64_t R, int64_t C, int64_t U,
+                  int64_t chunk, int mode, uint64_t seed, uint8_t* buf, int64_t* nslots_out);
+}
+
 extern "C" {
 
 // Transposes B matrices of R x C cells of U bytes in buf; mode 0 sequential,
 // 1 reversed, 2 shuffled, 3 interleaved.  Returns segments, or -1.
 int64_t ch_run(int64_t B, int64_t R, int64_t C, int64_t U, int64_t lmax, int64_t chunk, int mode,
                uint64_t seed, uint8_t* buf, int64_t* nslots_out) {
-  auto tab = eithot::build_cycle_table(R, C, lmax);
+  return run_table(eithot::build_cycle_table(R, C, lmax), B, R, C, U, chunk, mode, seed, buf,
+                   nslots_out);
+}
+
+// The composition of nmaps cell maps (bc, R, C, m each) over N cells of U
+// bytes, in one cycle pass.
+int64_t ch_run_maps(int nmaps, const int64_t* maps, int64_t N, int64_t U, int64_t lmax,
+                    int64_t chunk, int mode, uint64_t seed, uint8_t* buf) {
+  std::vector<eithot::CycleMap> mv;
+  for (int k = 0; k < nmaps; ++k)
+    mv.push_back(eithot::CycleMap{maps[4 * k], maps[4 * k + 1], maps[4 * k + 2], maps[4 * k + 3]});
+  return run_table(eithot::build_cycle_table_maps(mv, N, lmax), 1, N, 1, U, chunk, mode, seed, buf,
+                   nullptr);
+}
+
+}  // extern "C"
+
+namespace {
+int64_t run_table(std::shared_ptr<CycleTable> tab, int64_t B, int64_t R, int64_t C, int64_t U,
+                  int64_t chunk, int mode, uint64_t seed, uint8_t* buf, int64_t* nslots_out) {
   Ctx cx{buf, tab.get(), (uint64_t)R, (uint64_t)C, (uint64_t)U, (uint64_t)(R * C * U), {}};
   // boundary save pass
   const uint64_t ns = tab->slots.size();
@@ -122,6 +155,9 @@ int64_t ch_run(int64_t B, int64_t R, int64_t C, int64_t U, int64_t lmax, int64_t
   if (nslots_out) *nslots_out = (int64_t)ns;
   return (int64_t)tab->segs.size();
 }
+}  // namespace
+
+extern "C" {
 
 // Every position that is not a fixed point is written by exactly one
 // segment; segments never exceed lmax.  Returns 0 when the table is sound.

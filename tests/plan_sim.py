@@ -26,6 +26,15 @@ def simulate(buf, ops):
             view[o["base"]:o["base"] + nb] = blk[:, o["table"], :].reshape(-1)
             i += 1
             continue
+        if o["kind"] == 4 and "maps" in o:  # cycle following: its cell maps in pass order
+            nb = o["B"] * o["R"] * o["C"] * o["U"]
+            blk = np.ascontiguousarray(view[o["base"]:o["base"] + nb])
+            cells = nb // o["U"]
+            for bc, R, C, m in o["maps"]:
+                assert O.transpose2d_direct(blk, cells // bc, R, C, m * o["U"]) == 0
+            view[o["base"]:o["base"] + nb] = blk
+            i += 1
+            continue
         if o["kind"] in (1, 4):  # tile, or the same transpose by cycle following
             sub = view[o["base"]:]
             rc = O.transpose2d_direct(np.ascontiguousarray(sub), 0, 0, 0, 1)  # no-op warmup

@@ -27,6 +27,9 @@ def ch():
     L.ch_run.argtypes = [ctypes.c_int64] * 6 + [ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p,
                                                 ctypes.POINTER(ctypes.c_int64)]
     L.ch_check.argtypes = [ctypes.c_int64] * 3
+    L.ch_run_maps.restype = ctypes.c_int64
+    L.ch_run_maps.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int64)] + [ctypes.c_int64] * 4 + \
+        [ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p]
     return L
 
 
@@ -54,3 +57,22 @@ def test_cycle_emulation_equals_transpose(ch, mode, B, R, C, U, lmax, chunk):
     segs = ch.ch_run(B, R, C, U, lmax, chunk, mode, 5, b.ctypes.data, ctypes.byref(ns))
     assert segs >= 0
     assert np.array_equal(b, want), (B, R, C, U, lmax, mode, segs, ns.value)
+
+
+@pytest.mark.parametrize("mode", [0, 3])
+@pytest.mark.parametrize("maps,U,lmax", [
+    ([(96, 8, 12, 1), (96, 4, 3, 8)], 16, 7),             # two transposes of one 96-cell block
+    ([(30, 5, 6, 1), (60, 2, 3, 10), (120, 4, 30, 1)], 8, 1 << 30),
+    ([(64, 64, 1, 1), (64, 8, 8, 1)], 32, 5),             # identity map, then a square one
+    ([(240, 16, 15, 1), (240, 15, 16, 1)], 16, 11)])      # a transpose and its inverse
+def test_composite_cycle_maps_equal_sequential_transposes(ch, mode, maps, U, lmax):
+    N = max(bc for bc, _, _, _ in maps) * 2
+    vol = N * U
+    a = O.fill(vol, 1, 23, 1)
+    want = a.copy()
+    for bc, R, C, m in maps:  # the passes in order, each a batched transpose
+        assert O.transpose2d_direct(want, N // bc, R, C, m * U) == 0
+    flat = (ctypes.c_int64 * (4 * len(maps)))(*[v for mp in maps for v in mp])
+    b = a.copy()
+    assert ch.ch_run_maps(len(maps), flat, N, U, lmax, U, mode, 3, b.ctypes.data) >= 0
+    assert np.array_equal(b, want)

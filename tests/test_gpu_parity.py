@@ -87,6 +87,15 @@ def test_cycle_following_passes(torch_cuda):
         E.set_tuning("cycle_variant", default)
 
 
+def test_composite_cycle_passes(torch_cuda):
+    # consecutive whole-cell passes composed into one cycle-following pass
+    from test_planner import COMPOSITE_CASES
+    for dims, perm, elem in COMPOSITE_CASES + [([8, 40, 48, 9, 32, 16], [5, 3, 6, 2, 4, 1], 4)]:
+        got, want, peak = run_case(torch_cuda, dims, perm, elem, seed=10, alpha=0.2)
+        assert_bitexact(got, want, f"{dims} {perm} elem {elem}")
+        assert peak <= 0.2 * got.nbytes
+
+
 def test_fused_block_passes(torch_cuda):
     # nested contiguous tile passes fused into one LDS block permutation
     # (padded and unpadded LDS layouts, 1/2/4/8-byte cells), and the same

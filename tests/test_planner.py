@@ -155,6 +155,27 @@ def test_fused_block_plans_execute_correctly():
     assert sorted(blk[0]["table"].tolist()) == list(range(48 * 40 * 8))
 
 
+COMPOSITE_CASES = [([12, 2, 25, 14, 53, 3], [3, 6, 4, 1, 5, 2], 8),
+                   ([2, 19, 45, 5, 3, 11], [5, 4, 2, 6, 3, 1], 8),
+                   ([12, 9, 14, 28, 53], [2, 1, 5, 4, 3], 8),
+                   ([2, 4, 2, 57, 55, 7], [2, 1, 4, 6, 5, 3], 8),
+                   ([24, 15, 4, 6, 2, 59], [6, 1, 4, 2, 5, 3], 4),
+                   ([2, 42, 2, 11, 11], [5, 2, 1, 4, 3], 8)]
+
+
+@pytest.mark.parametrize("dims,perm1,elem", COMPOSITE_CASES)
+def test_composite_cycle_plans_execute_correctly(dims, perm1, elem):
+    # consecutive whole-cell passes composed into one cycle-following pass
+    ops, scratch = E.plan_describe(dims, perm1, elem, 1, 0.2)
+    assert any(o["kind"] == 4 and len(o["maps"]) > 1 for o in ops), ops
+    vol = int(np.prod(dims))
+    assert scratch <= 0.2 * vol * elem
+    a = O.fill(vol, elem, 13, 1)
+    b = a.copy()
+    simulate(b, ops)
+    assert np.array_equal(b, O.permute(a, dims, [p - 1 for p in perm1], elem))
+
+
 def test_partition_plan_executes_correctly():
     for dims, perm1, nsub in (([152, 150], [2, 1], 4), ([6, 400, 3], [3, 2, 1], 8),
                               ([96, 40, 5], [2, 3, 1], 4)):
