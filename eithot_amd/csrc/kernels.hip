@@ -1086,6 +1086,7 @@ struct CycArgs {
   const uint32_t* slots;
   char* boundary;
   uint32_t nseg, nslots, nmaps, chunk_bytes;
+  uint32_t lane_log2;  // log2 of the lanes per item (6: one item per wave)
   uint64_t B, items, chunks;
   CycMapDev maps[4];
   FD64 fchunks, fnseg;
@@ -1158,8 +1159,11 @@ __device__ __forceinline__ void st_nt(char* p, const Word& w) {
 template <typename Word, int D, int NT = 0>
 __global__ __launch_bounds__(256) void k_cycle(CycArgs a) {
   constexpr uint64_t w = sizeof(Word);
-  const uint32_t lane = threadIdx.x & 63;
-  const uint64_t item = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  // an item is 2^lane_log2 consecutive lanes: a whole wave, or a group of
+  // lanes when cells are smaller than 64 words (several items per wave)
+  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lane = (uint32_t)(gtid & ((1u << a.lane_log2) - 1));
+  const uint64_t item = gtid >> a.lane_log2;
   if (item >= a.items) return;
   const uint64_t rest = a.fchunks.div(item);
   const uint64_t chunk = item - rest * a.chunks;
@@ -1305,8 +1309,10 @@ hipError_t launch_cycle_w(CycArgs a, int variant, hipStream_t stream) {
     const uint64_t blocks = std::min<uint64_t>((words + 255) / 256, 256ull * 64);
     hipLaunchKernelGGL((k_cycle_save<Word>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
   }
-  const uint64_t blocks = (a.items + 3) / 4;
-  if (variant == 1)
+  const uint64_t blocks = ((a.items << a.lane_log2) + 255) / 256;
+  if (a.lane_log2 < 6)  // small cells: several items per wave
+    hipLaunchKernelGGL((k_cycle<Word, 8>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
+  else if (variant == 1)
     hipLaunchKernelGGL((k_cycle_pipe<Word, 4, 2>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
   else if (variant == 2)
     hipLaunchKernelGGL((k_cycle_pipe<Word, 4, 1>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
@@ -1361,13 +1367,17 @@ hipError_t launch_cycle(const Op& op, char* data, char* scratch, int64_t scratch
     a.maps[k].fm.init((uint32_t)M.m);
     a.maps[k].fR.init((uint32_t)M.R);
   }
-  a.chunk_bytes = (uint32_t)(64 * w * cycle_vwords(variant));
+  // cells smaller than a wave's row of words: 2^lane_log2 lanes per cell
+  a.lane_log2 = 6;
+  while (a.lane_log2 > 0 && ((uint64_t)w << (a.lane_log2 - 1)) >= (uint64_t)op.U) --a.lane_log2;
+  if (a.lane_log2 < 6) variant = 0;
+  a.chunk_bytes = (uint32_t)((w << a.lane_log2) * cycle_vwords(variant));
   a.B = op.B;
   a.chunks = (op.U + a.chunk_bytes - 1) / a.chunk_bytes;
   a.items = a.B * a.nseg * a.chunks;
   a.fchunks.init(a.chunks);
   a.fnseg.init(a.nseg);
-  if ((a.items + 3) / 4 >= (1ull << 31)) return hipErrorInvalidValue;
+  if (((a.items << a.lane_log2) + 255) / 256 >= (1ull << 31)) return hipErrorInvalidValue;
   switch (w) {
     case 16: return launch_cycle_w<W16>(a, variant, stream);
     case 8: return launch_cycle_w<uint64_t>(a, variant, stream);

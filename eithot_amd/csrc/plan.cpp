@@ -212,57 +212,92 @@ int64_t Op::num_footprints() const {
   return 0;
 }
 
+namespace {
+// x / d for x, d < 2^32 by one multiply: with M = floor((2^64 - 1) / d) + 1,
+// floor(x * M / 2^64) = floor(x / d) because x * d < 2^64.
+struct HostDiv {
+  uint64_t d = 1, M = 0;
+  explicit HostDiv(uint64_t dv) : d(dv ? dv : 1), M(d > 1 ? ~0ull / d + 1 : 0) {}
+  uint64_t div(uint64_t x) const {
+    return d == 1 ? x : (uint64_t)(((unsigned __int128)x * M) >> 64);
+  }
+};
+
+struct MapDiv {
+  HostDiv bc, m, R;
+  uint64_t C;
+};
+
+std::mutex g_cyc_mu;
+std::map<std::vector<int64_t>, std::shared_ptr<CycleTable>> g_cyc_cache;  // recent tables
+}  // namespace
+
 std::shared_ptr<CycleTable> build_cycle_table_maps(const std::vector<CycleMap>& maps, int64_t N,
                                                    int64_t lmax) {
+  std::vector<int64_t> key = {N, lmax};
+  for (const CycleMap& M : maps) key.insert(key.end(), {M.bc, M.R, M.C, M.m});
+  {
+    std::lock_guard<std::mutex> lk(g_cyc_mu);
+    auto it = g_cyc_cache.find(key);
+    if (it != g_cyc_cache.end()) return it->second;
+  }
   auto t = std::make_shared<CycleTable>();
   t->maps = maps;
   t->lmax = lmax;
   if (N <= 0 || (uint64_t)N >= (1ull << 32) || lmax < 1 || maps.empty()) return t;
+  std::vector<MapDiv> md;
+  for (const CycleMap& M : maps)
+    md.push_back(MapDiv{HostDiv((uint64_t)M.bc), HostDiv((uint64_t)M.m), HostDiv((uint64_t)M.R),
+                        (uint64_t)M.C});
   auto src = [&](uint64_t p) {
-    for (size_t k = maps.size(); k-- > 0;) {
-      const CycleMap& M = maps[k];
-      const uint64_t b = p / (uint64_t)M.bc, q = p - b * (uint64_t)M.bc;
-      const uint64_t c = q / (uint64_t)M.m, x = q - c * (uint64_t)M.m;
-      const uint64_t j = c / (uint64_t)M.R, i = c - j * (uint64_t)M.R;
-      p = b * (uint64_t)M.bc + (i * (uint64_t)M.C + j) * (uint64_t)M.m + x;
+    for (size_t k = md.size(); k-- > 0;) {
+      const MapDiv& M = md[k];
+      const uint64_t b = M.bc.div(p), q = p - b * M.bc.d;
+      const uint64_t c = M.m.div(q), x = q - c * M.m.d;
+      const uint64_t j = M.R.div(c), i = c - j * M.R.d;
+      p = b * M.bc.d + (i * M.C + j) * M.m.d + x;
     }
     return p;
   };
-  std::vector<uint8_t> seen((size_t)N, 0);
+  // one walk per cycle over a visited bitset; a cycle longer than lmax is
+  // cut into segments of lmax positions (the last one shorter), each
+  // starting at a recorded position with a boundary slot
+  std::vector<uint64_t> seen(((size_t)N + 63) / 64, 0);
+  std::vector<uint32_t> starts;
   for (uint64_t p0 = 0; p0 < (uint64_t)N; ++p0) {
-    if (seen[p0]) continue;
-    uint64_t k = 0, q = p0;
+    if (seen[p0 >> 6] >> (p0 & 63) & 1) continue;
+    starts.clear();
+    uint64_t k = 0, q = p0, left = 0;
     do {
-      seen[q] = 1;
+      if (left == 0) {
+        starts.push_back((uint32_t)q);
+        left = (uint64_t)lmax;
+      }
+      --left;
+      seen[q >> 6] |= 1ull << (q & 63);
       q = src(q);
       ++k;
     } while (q != p0);
     if (k == 1) continue;
-    if ((int64_t)k <= lmax) {
+    if (starts.size() == 1) {
       t->segs.push_back({(uint32_t)p0, (uint32_t)k, -1, 0});
       continue;
     }
-    // cut into n nearly equal segments; segment i starts at q_i and takes
-    // its last cell from the slot of segment i+1 (cyclically)
-    const uint64_t n = (k + (uint64_t)lmax - 1) / (uint64_t)lmax;
-    const uint32_t first_slot = (uint32_t)t->slots.size();
-    q = p0;
-    for (uint64_t i = 0; i < n; ++i) {
-      const uint64_t len = k / n + (i < k % n ? 1 : 0);
-      t->slots.push_back((uint32_t)q);
-      t->segs.push_back({(uint32_t)q, (uint32_t)len,
-                         (int32_t)(first_slot + (uint32_t)((i + 1) % n)), 0});
-      for (uint64_t s = 0; s < len; ++s) q = src(q);
+    const uint32_t n = (uint32_t)starts.size(), first_slot = (uint32_t)t->slots.size();
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint64_t len = i + 1 < n ? (uint64_t)lmax : k - (uint64_t)(n - 1) * (uint64_t)lmax;
+      t->slots.push_back(starts[i]);
+      t->segs.push_back({starts[i], (uint32_t)len, (int32_t)(first_slot + (i + 1) % n), 0});
     }
   }
+  std::lock_guard<std::mutex> lk(g_cyc_mu);
+  if (g_cyc_cache.size() >= 16) g_cyc_cache.clear();
+  g_cyc_cache[key] = t;
   return t;
 }
 
 std::shared_ptr<CycleTable> build_cycle_table(int64_t R, int64_t C, int64_t lmax) {
-  auto t = build_cycle_table_maps({CycleMap{R * C, R, C, 1}}, R * C, lmax);
-  t->R = R;
-  t->C = C;
-  return t;
+  return build_cycle_table_maps({CycleMap{R * C, R, C, 1}}, R * C, lmax);
 }
 
 namespace {
@@ -401,7 +436,8 @@ CatPlan plan_catanzaro(int64_t R, int64_t C, int64_t U, const DeviceProfile& pro
   p.row_staged = false;
   p.row_us = pick_slice(p.n, U, prof.lds_max, prof.min_segment);
   if (!p.row_us) fallback(p.n, p.row_us, p.row_staged);
-  const double col_cost = p.col_staged ? 2.0 : 1.0, row_cost = p.row_staged ? 2.0 : 1.0;
+  const double col_cost = p.col_staged ? 2.0 * prof.staged_mult : 1.0,
+               row_cost = p.row_staged ? 2.0 * prof.staged_mult : 1.0;
   p.cost = row_cost + col_cost * (p.npass - 1) + narrow_cost;
   // a staged pass needs at least one footprint of scratch
   const int64_t col_fp = p.m * p.col_W * p.col_us, row_fp = p.n * p.row_us;
@@ -469,7 +505,9 @@ constexpr double kCostSplit = 0.001;
 // Cycle-following moves whole cells (of at least cycle_min_cell bytes), one
 // read and one write each; smaller cells waste HBM bursts.
 constexpr uint64_t kCycleMaxCells = 1ull << 26;
-inline double cycle_cost(int64_t U) { return U >= 1024 ? 1.6 : 1.75; }
+constexpr double kCostCycle = 1.15;  // 4.6-5.3 TB/s for cells of 128 B to 384 KB
+constexpr double kCostBlock = 1.1;   // fused block permutation
+inline double cycle_cost(int64_t) { return kCostCycle; }
 
 struct Choice {
   int how = HOW_NONE;
@@ -515,9 +553,9 @@ struct Lowerer {
       ch.cost = cycle_cost(U);
     }
     // whole matrices (or slices) through scratch: copy out, tiled transpose back
-    if (ch.cost > kCostTileStaged) {
+    if (ch.cost > kCostTileStaged * prof.staged_mult) {
       const int64_t us = pick_staged_tile_slice(R, C, U, scratch_budget, prof);
-      const double cost = us == U ? kCostTileStaged : kCostTileStagedSliced;
+      const double cost = (us == U ? kCostTileStaged : kCostTileStagedSliced) * prof.staged_mult;
       if (us && cost < ch.cost) {
         ch.how = HOW_TILE_STAGED;
         ch.cost = cost;
@@ -553,12 +591,6 @@ struct Lowerer {
     return ch;
   }
 
-  std::shared_ptr<const CycleTable> cycle_table_for(int64_t B, int64_t R, int64_t C, int64_t U) {
-    auto t = choose_cycle_table({CycleMap{R * C, R, C, 1}}, R * C, B, U, scratch_budget);
-    t->R = R;
-    t->C = C;
-    return t;
-  }
 
   void emit(int64_t B, int64_t R, int64_t C, int64_t U, uint64_t base, int stage, int depth,
             std::vector<Op>& ops) {
@@ -602,7 +634,7 @@ struct Lowerer {
         op.C = C;
         op.U = U;
         op.us = U;
-        op.cyc = cycle_table_for(B, R, C, U);
+        op.cmaps = {CycleMap{R * C, R, C, 1}};  // table built once the plan is chosen
         ops.push_back(op);
         return;
       }
@@ -712,7 +744,7 @@ int tuning_set(const char* key, int64_t value) {
   if (!f || value < 0) return EITHOT_ERR_INVALID;
   const bool threads = f == &t.threads || f == &t.threads_big;
   if (threads && (value < 64 || value > 512 || value % 64)) return EITHOT_ERR_INVALID;
-  if (f == &t.reorder && value > 2) return EITHOT_ERR_INVALID;
+  if (f == &t.reorder && value > 3) return EITHOT_ERR_INVALID;
   if (f == &t.cycle_variant && value > 7) return EITHOT_ERR_INVALID;
   if (f == &t.lds_target && value < 1024) return EITHOT_ERR_INVALID;
   *f = value;
@@ -861,24 +893,26 @@ std::shared_ptr<BlockTable> compose_tiles(const std::vector<Op>& ops, size_t i0,
 // bytes over the same bytes (cycle passes and whole-unit tiles) compose into
 // one cycle pass over cells of their common size, when at least one of them
 // is a cycle pass.
-void fuse_cycles(std::vector<Op>& ops, const DeviceProfile& prof, int64_t budget) {
+void fuse_cycles(std::vector<Op>& ops, const DeviceProfile& prof) {
   if (prof.cycle_min_cell <= 0) return;
   auto cellperm = [&](const Op& o) {
-    return (o.kind == OP_CYCLE && o.cyc && o.cyc->maps.size() == 1) ||
+    return (o.kind == OP_CYCLE && !o.cmaps.empty()) ||
            (o.kind == OP_TILE && !o.staged && o.us == o.U);
   };
+  auto nmaps = [](const Op& o) { return o.kind == OP_CYCLE ? o.cmaps.size() : (size_t)1; };
   std::vector<Op> out;
   size_t i = 0;
   while (i < ops.size()) {
-    size_t j = i;
+    size_t j = i, maps_n = 0;
     int64_t g = 0;
     int cycles = 0;
-    while (j < ops.size() && j - i < 4 && cellperm(ops[j]) && ops[j].base == ops[i].base &&
-           ops[j].bytes() == ops[i].bytes()) {
+    while (j < ops.size() && cellperm(ops[j]) && maps_n + nmaps(ops[j]) <= 4 &&
+           ops[j].base == ops[i].base && ops[j].bytes() == ops[i].bytes()) {
       const int64_t g2 = gcd64(g, ops[j].U);
       if (g2 < prof.cycle_min_cell || ops[j].bytes() / (uint64_t)g2 > kCycleMaxCells) break;
       g = g2;
       cycles += ops[j].kind == OP_CYCLE;
+      maps_n += nmaps(ops[j]);
       ++j;
     }
     if (j - i < 2 || !cycles) {
@@ -886,27 +920,48 @@ void fuse_cycles(std::vector<Op>& ops, const DeviceProfile& prof, int64_t budget
       ++i;
       continue;
     }
+    // every map re-expressed in g-byte cells, in pass order
     std::vector<CycleMap> maps;
     for (size_t q = i; q < j; ++q) {
       const Op& o = ops[q];
-      const int64_t m = o.U / g;
-      maps.push_back(CycleMap{o.R * o.C * m, o.R, o.C, m});
+      const int64_t f = o.U / g;
+      if (o.kind == OP_TILE)
+        maps.push_back(CycleMap{o.R * o.C * f, o.R, o.C, f});
+      else
+        for (const CycleMap& M : o.cmaps) maps.push_back(CycleMap{M.bc * f, M.R, M.C, M.m * f});
     }
-    const int64_t N = (int64_t)(ops[i].bytes() / (uint64_t)g);
     Op c;
     c.kind = OP_CYCLE;
     c.stage = ops[i].stage;
     c.base = ops[i].base;
     c.B = 1;
-    c.R = N;
+    c.R = (int64_t)(ops[i].bytes() / (uint64_t)g);
     c.C = 1;
     c.U = g;
     c.us = g;
-    c.cyc = choose_cycle_table(maps, N, 1, g, budget);
+    c.cmaps = maps;
     out.push_back(c);
     i = j;
   }
   ops.swap(out);
+}
+
+// Relative time of a plan under the pass costs above.
+double plan_cost(const Plan& plan) {
+  double c = 0;
+  for (const Op& o : plan.ops) {
+    switch (o.kind) {
+      case OP_TILE:
+        c += o.staged ? (o.us == o.U ? kCostTileStaged : kCostTileStagedSliced)
+                      : (o.us == o.U ? kCostTile : kCostTileSliced);
+        break;
+      case OP_COL:
+      case OP_ROW: c += o.staged ? 2.0 : 1.0; break;
+      case OP_CYCLE: c += kCostCycle; break;
+      case OP_BLOCK: c += kCostBlock; break;
+    }
+  }
+  return c;
 }
 
 void fuse_tiles(std::vector<Op>& ops, const DeviceProfile& prof) {
@@ -1024,36 +1079,83 @@ static int make_plan_impl(const int64_t* dims_in, const int* perm1, int rank, si
   }
   for (const Op& op : plan.ops)
     if (op.kind == 0) return EITHOT_ERR_ALPHA;
-  fuse_cycles(plan.ops, prof, budget);
+  fuse_cycles(plan.ops, prof);
   fuse_tiles(plan.ops, prof);
-  for (const Op& op : plan.ops) {
+  return EITHOT_OK;
+}
+
+// Cycle tables of the chosen plan, then its scratch and table bytes.
+static void finalize_plan(Plan& plan, double alpha, int64_t reserve, const DeviceProfile& prof) {
+  const int64_t budget = std::max<int64_t>(
+      (int64_t)(alpha * (double)plan.tensor_bytes) - reserve, (int64_t)1);
+  plan.scratch_bytes = 0;
+  plan.table_bytes = 0;
+  for (Op& op : plan.ops) {
+    if (op.kind == OP_CYCLE && !op.cyc) {
+      op.cyc = choose_cycle_table(op.cmaps, op.R * op.C, op.B, op.U, budget);
+    }
     plan.scratch_bytes = std::max(plan.scratch_bytes, staged_scratch(op, budget, prof));
     if (op.cyc) plan.table_bytes += op.cyc->bytes();
     if (op.blk) plan.table_bytes += op.blk->bytes();
   }
-  return EITHOT_OK;
 }
 
 int make_plan(const int64_t* dims_in, const int* perm1, int rank, size_t elem, int n_sub,
               double alpha, const DeviceProfile& prof, int reorder, Plan& plan) {
-  int rc = make_plan_impl(dims_in, perm1, rank, elem, n_sub, alpha, prof, reorder, 0, plan);
-  if (rc || plan.table_bytes == 0) return rc;
+  // Search: decomposer order (pre/post, or the one asked for) x how hard
+  // staged passes are avoided; cycle passes that later compose with their
+  // neighbours make a split lowering cheaper than a staged one, which only
+  // the whole-plan cost shows.
+  std::vector<int> orders;
+  if (reorder == 0)
+    orders = {REORDER_MIX, REORDER_PRE, REORDER_POST};  // ties keep the mix rule's choice
+  else
+    orders = {reorder == 3 ? (int)REORDER_MIX : reorder};
+  int rc = EITHOT_ERR_INVALID;
+  double best_cost = 1e300;
+  int best_order = orders[0];
+  double best_mult = 1.0;
+  for (int o : orders)
+    for (double mult : {1.0, 3.0}) {
+      DeviceProfile p2 = prof;
+      p2.staged_mult = mult;
+      Plan q;
+      const int r = make_plan_impl(dims_in, perm1, rank, elem, n_sub, alpha, p2, o, 0, q);
+      if (r) {
+        if (rc) rc = r;
+        continue;
+      }
+      const double c = plan_cost(q);
+      if (rc || c < best_cost - 1e-9) {
+        best_cost = c;
+        best_order = o;
+        best_mult = mult;
+        plan = std::move(q);
+      }
+      rc = EITHOT_OK;
+    }
+  if (rc) return rc;
+  DeviceProfile pb = prof;
+  pb.staged_mult = best_mult;
+  finalize_plan(plan, alpha, 0, pb);
+  if (plan.table_bytes == 0) return EITHOT_OK;
   // The pass tables count as extra memory too.  When they push the plan
   // over its budget: plan again with their bytes kept back from the
   // scratch budget; then without cycle following; then without fusion;
   // keep whichever needs the least extra memory.
   const int64_t budget = (int64_t)(alpha * (double)plan.tensor_bytes);
   auto extra = [](const Plan& p) { return p.scratch_bytes + p.table_bytes; };
-  if (extra(plan) <= budget) return rc;
+  if (extra(plan) <= budget) return EITHOT_OK;
   Plan best = plan;
-  DeviceProfile p2 = prof;
   for (int attempt = 0; attempt < 3; ++attempt) {
     int64_t reserve = 0;
     if (attempt == 0) reserve = plan.table_bytes;
-    if (attempt == 1) p2.cycle_min_cell = 0;
-    if (attempt == 2) p2.fuse_max_block = 0;
+    if (attempt == 1) pb.cycle_min_cell = 0;
+    if (attempt == 2) pb.fuse_max_block = 0;
     Plan q;
-    if (make_plan_impl(dims_in, perm1, rank, elem, n_sub, alpha, p2, reorder, reserve, q)) continue;
+    if (make_plan_impl(dims_in, perm1, rank, elem, n_sub, alpha, pb, best_order, reserve, q))
+      continue;
+    finalize_plan(q, alpha, reserve, pb);
     if (extra(q) < extra(best)) best = q;
     if (extra(best) <= budget) break;
   }

@@ -68,7 +68,7 @@ struct CycleMap {
   int64_t bc, R, C, m;
 };
 struct CycleTable {
-  int64_t R = 0, C = 0, lmax = 0;
+  int64_t lmax = 0;
   // maps applied in pass order; the pass's source of position p is
   // src_1(src_2(...src_k(p))) (gather composition)
   std::vector<CycleMap> maps;
@@ -121,7 +121,8 @@ struct Op {
   // Catanzaro constants of the m x n grid (m = R, n = C of this op's grid)
   int64_t m = 1, n = 1, c = 1, a = 1, b = 1, k = 0, binv = 0;
 
-  std::shared_ptr<const CycleTable> cyc;  // OP_CYCLE: segments shared by all B matrices
+  std::vector<CycleMap> cmaps;            // OP_CYCLE: cell maps (in units of U-byte cells)
+  std::shared_ptr<const CycleTable> cyc;  // OP_CYCLE: segments shared by all B regions
   std::shared_ptr<const BlockTable> blk;  // OP_BLOCK: B blocks of R cells of U bytes
 
   int64_t footprint_units() const;   // units staged per footprint
@@ -141,10 +142,11 @@ struct DeviceProfile {
   int64_t tile_fp_target = 64 * 1024;
   int64_t slice_min = 128;
   int64_t stage_chunk = 0;        // > 0: bytes of scratch one staged group may fill
-  int64_t cycle_min_cell = 512;   // smallest cell moved by cycle following (0: off)
+  int64_t cycle_min_cell = 128;   // smallest cell moved by cycle following (0: off)
   int64_t fuse_max_block = 64 * 1024;  // largest fused block permutation (0: no fusion)
   int64_t block_digits = 1;       // fused blocks index cells arithmetically when they can
-  int reorder = 0;                // decomposer order (Reorder)
+  int reorder = 0;                // 0 cheapest of pre/post (mix breaks ties), 1 pre, 2 post, 3 mix
+  double staged_mult = 1.0;       // planner-internal: weight of staged passes in the search
 };
 
 // Process-wide tuning knobs (eithot_set_tuning); applied over the hardware
@@ -154,7 +156,7 @@ struct Tuning {
   int64_t tile_fp_target = 64 * 1024;
   int64_t slice_min = 128;
   int64_t stage_chunk = 0;
-  int64_t cycle_min_cell = 512;
+  int64_t cycle_min_cell = 128;
   int64_t fuse_max_block = 64 * 1024;
   int64_t block_digits = 1;
   int64_t threads = 256;
@@ -183,7 +185,10 @@ struct Plan {
 // Full plan.  perm1 is 1-based.  alpha bounds the scratch (scratch <= alpha *
 // tensor bytes, at least one footprint); n_sub > 1 requests the reference's
 // sub-tensor partitioning along the largest dim when that dim exceeds
-// floor(32/alpha) and divides evenly.  Returns 0 or a negative EITHOT_ERR_*.
+// floor(32/alpha) and divides evenly.  reorder: 0 plans both decomposer
+// orders and keeps the cheaper (the reference's mix rule breaks ties), 1
+// pre-order, 2 post-order, 3 the mix rule.  Returns 0 or a negative
+// EITHOT_ERR_*.
 int make_plan(const int64_t* dims, const int* perm1, int rank, size_t elem, int n_sub,
               double alpha, const DeviceProfile& prof, int reorder, Plan& plan);
 

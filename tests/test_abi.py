@@ -39,7 +39,7 @@ def test_library_is_a_gfx950_hip_build():
 
 def test_error_message_thread_local_and_planning_without_gpu():
     n, a, x = E.estimate([2, 4096, 1024, 2], [1, 3, 2, 4], 4, 0.05)
-    assert n == 1 and x == 0 and a == 0
+    assert n == 1 and x <= 0.05 * 2 * 4096 * 1024 * 2 * 4 and a <= 0.05
     rc = E.lib().eithot_transpose(None, E.i64([2]), E.i32([1]), 1, 4, 1, 0.05)
     assert rc == -1
     assert b"null" in E.lib().eithot_last_error()

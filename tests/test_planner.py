@@ -189,13 +189,25 @@ def test_partition_plan_executes_correctly():
 
 
 def test_config_plans():
-    # config 1: three LDS Catanzaro passes, no scratch
+    # config 1: at most the three Catanzaro passes of the reference's lowering,
+    # scratch within 5 % of 64 MiB
     ops, scratch = E.plan_describe([2, 4096, 1024, 2], [1, 3, 2, 4], 4, 1, 0.05)
+    assert len(ops) <= 3 and not any(o["staged"] for o in ops)
+    assert scratch <= 0.05 * 2 * 4096 * 1024 * 2 * 4
+    # with the reference's lowering only (no cycle following, no fusion):
+    # three LDS Catanzaro passes, no scratch
+    old = {k: E.set_tuning(k, 0) for k in ("cycle_min_cell", "fuse_max_block")}
+    try:
+        ops, scratch = E.plan_describe([2, 4096, 1024, 2], [1, 3, 2, 4], 4, 1, 0.05)
+    finally:
+        for k, v in old.items():
+            E.set_tuning(k, v)
     assert [(o["kind"], o["map"], o["staged"]) for o in ops] == [(2, 1, 0), (3, 5, 0), (2, 3, 0)]
     assert scratch == 0
-    # config 2: one pass per reference stage, scratch within 5 % of 4.53 GB
+    # config 2: fewer passes than the reference's five stages, none staged
+    # (forward), scratch within 5 % of 4.53 GB
     ops, scratch = E.plan_describe([32, 48, 64, 40, 36, 8], [6, 4, 2, 5, 1, 3], 4, 1, 0.05)
-    assert len(ops) <= 5 and all(o["kind"] in (1, 4, 5) for o in ops)
+    assert len(ops) <= 4 and all(o["kind"] in (1, 4, 5) and not o["staged"] for o in ops)
     assert scratch <= 0.05 * 32 * 48 * 64 * 40 * 36 * 8 * 4
     # config 3: scratch within 5 % of 171.8 GB
     ops, scratch = E.plan_describe([2, 163840, 65536, 2], [1, 3, 2, 4], 4, 1, 0.05)
