@@ -449,7 +449,11 @@ CatPlan plan_catanzaro(int64_t R, int64_t C, int64_t U, const DeviceProfile& pro
   return p;
 }
 
-std::vector<int64_t> divisors(int64_t x) {
+const std::vector<int64_t>& divisors(int64_t x) {
+  thread_local std::map<int64_t, std::vector<int64_t>> memo;
+  auto it = memo.find(x);
+  if (it != memo.end()) return it->second;
+  if (memo.size() > 4096) memo.clear();
   std::vector<int64_t> d;
   for (int64_t q = 1; q * q <= x; ++q)
     if (x % q == 0) {
@@ -457,7 +461,7 @@ std::vector<int64_t> divisors(int64_t x) {
       if (q != x / q) d.push_back(x / q);
     }
   std::sort(d.begin(), d.end());
-  return d;
+  return memo[x] = std::move(d);
 }
 
 // Segment length of a cycle pass over B regions of N cells of U bytes:
@@ -798,17 +802,16 @@ namespace {
 
 bool fusable_tile(const Op& o) { return o.kind == OP_TILE && !o.staged && o.us == o.U; }
 
-std::shared_ptr<BlockTable> compose_tiles(const std::vector<Op>& ops, size_t i0, size_t i1,
-                                          int64_t F, int64_t g, bool digits) {
+std::shared_ptr<BlockTable> compose_tiles(const std::vector<int64_t>& parts, int64_t F, int64_t g,
+                                          bool digits) {
   auto t = std::make_shared<BlockTable>();
   const int64_t n = F / g;
   t->cells = n;
   t->cell_bytes = g;
   std::vector<uint32_t> cur(n), nxt(n);
   for (int64_t i = 0; i < n; ++i) cur[i] = (uint32_t)i;
-  for (size_t q = i0; q < i1; ++q) {
-    const Op& o = ops[q];
-    const int64_t k = o.U / g, R = o.R, C = o.C, fb = R * C * k;
+  for (size_t q = 0; q + 2 < parts.size(); q += 3) {
+    const int64_t R = parts[q], C = parts[q + 1], k = parts[q + 2] / g, fb = R * C * k;
     for (int64_t b0 = 0; b0 < n; b0 += fb)
       for (int64_t j = 0; j < C; ++j)
         for (int64_t i = 0; i < R; ++i)
@@ -860,7 +863,7 @@ std::shared_ptr<BlockTable> compose_tiles(const std::vector<Op>& ops, size_t i0,
       double tot = 0;
       int64_t cnt = 0;
       const int64_t chunks = n / V;
-      for (int64_t t0 = 0; t0 < chunks && t0 < 64 * 64; t0 += 64)
+      for (int64_t t0 = 0; t0 < chunks && t0 < 16 * 64; t0 += 64)
         for (int64_t v = 0; v < V; ++v)
           for (int banks : {32, 64}) {
             for (int64_t h = 0; h < 64; h += banks) {
@@ -1001,7 +1004,7 @@ void fuse_tiles(std::vector<Op>& ops, const DeviceProfile& prof) {
     b.C = 1;
     b.U = g;
     b.us = g;
-    b.blk = compose_tiles(ops, i, j, F, g, prof.block_digits != 0);
+    for (size_t q = i; q < j; ++q) b.bparts.insert(b.bparts.end(), {ops[q].R, ops[q].C, ops[q].U});
     out.push_back(b);
     i = j;
   }
@@ -1013,7 +1016,7 @@ void fuse_tiles(std::vector<Op>& ops, const DeviceProfile& prof) {
 // `reserve`: bytes of the alpha budget kept back for the pass tables.
 static int make_plan_impl(const int64_t* dims_in, const int* perm1, int rank, size_t elem,
                           int n_sub, double alpha, const DeviceProfile& prof, int reorder,
-                          int64_t reserve, Plan& plan) {
+                          int64_t reserve, Plan& plan, Lowerer* shared = nullptr) {
   plan = Plan();
   if (rank < 1 || rank > 64 || !dims_in || !perm1) return EITHOT_ERR_INVALID;
   if (elem == 0 || elem > (1u << 20)) return EITHOT_ERR_ELEM;
@@ -1031,7 +1034,9 @@ static int make_plan_impl(const int64_t* dims_in, const int* perm1, int rank, si
   plan.stages = stages;
   const int64_t budget = std::max<int64_t>(
       (int64_t)(alpha * (double)plan.tensor_bytes) - reserve, (int64_t)1);
-  Lowerer L(prof, budget);
+  Lowerer own(prof, budget);
+  // a caller's lowerer (same profile and budget) keeps its memo across orders
+  Lowerer& L = shared && shared->scratch_budget == budget ? *shared : own;
 
   // Sub-tensor partitioning along the largest dim (Nd_transpose.cpp:265-300):
   // only when the caller asks for it and the dim exceeds floor(32/alpha).
@@ -1091,6 +1096,8 @@ static void finalize_plan(Plan& plan, double alpha, int64_t reserve, const Devic
   plan.scratch_bytes = 0;
   plan.table_bytes = 0;
   for (Op& op : plan.ops) {
+    if (op.kind == OP_BLOCK && !op.blk)
+      op.blk = compose_tiles(op.bparts, op.R * op.U, op.U, prof.block_digits != 0);
     if (op.kind == OP_CYCLE && !op.cyc) {
       op.cyc = choose_cycle_table(op.cmaps, op.R * op.C, op.B, op.U, budget);
     }
@@ -1107,20 +1114,47 @@ int make_plan(const int64_t* dims_in, const int* perm1, int rank, size_t elem, i
   // neighbours make a split lowering cheaper than a staged one, which only
   // the whole-plan cost shows.
   std::vector<int> orders;
-  if (reorder == 0)
-    orders = {REORDER_MIX, REORDER_PRE, REORDER_POST};  // ties keep the mix rule's choice
-  else
+  if (reorder == 0 && rank > 0 && rank <= 64 && dims_in && perm1) {
+    // the mix rule's order first, so that it wins ties; the other order only
+    // when its stages differ
+    std::vector<Stage> mix, pre, post;
+    decompose(dims_in, perm1, rank, REORDER_MIX, mix);
+    decompose(dims_in, perm1, rank, REORDER_PRE, pre);
+    decompose(dims_in, perm1, rank, REORDER_POST, post);
+    auto same = [](const std::vector<Stage>& a, const std::vector<Stage>& b) {
+      if (a.size() != b.size()) return false;
+      for (size_t i = 0; i < a.size(); ++i)
+        if (a[i].P != b[i].P || a[i].X != b[i].X || a[i].Y != b[i].Y || a[i].S != b[i].S)
+          return false;
+      return true;
+    };
+    const bool mix_is_pre = same(mix, pre);
+    orders = {mix_is_pre ? (int)REORDER_PRE : (int)REORDER_POST};
+    if (!same(pre, post)) orders.push_back(mix_is_pre ? (int)REORDER_POST : (int)REORDER_PRE);
+  } else {
     orders = {reorder == 3 ? (int)REORDER_MIX : reorder};
+  }
+  // one lowerer (and its memo) per staged weight, shared by the orders
+  uint64_t vol = 1;
+  bool vol_ok = rank > 0 && rank <= 64 && dims_in && elem > 0;
+  for (int i = 0; vol_ok && i < rank; ++i) {
+    if (dims_in[i] < 1 || vol > (UINT64_MAX / 16) / (uint64_t)dims_in[i]) vol_ok = false;
+    else vol *= (uint64_t)dims_in[i];
+  }
+  const int64_t budget0 =
+      vol_ok && alpha > 0 ? std::max<int64_t>((int64_t)(alpha * (double)(vol * elem)), 1) : 1;
   int rc = EITHOT_ERR_INVALID;
   double best_cost = 1e300;
   int best_order = orders[0];
   double best_mult = 1.0;
-  for (int o : orders)
-    for (double mult : {1.0, 3.0}) {
-      DeviceProfile p2 = prof;
-      p2.staged_mult = mult;
+  for (double mult : {1.0, 3.0}) {
+    DeviceProfile p2 = prof;
+    p2.staged_mult = mult;
+    Lowerer shared(p2, budget0);
+    for (int o : orders) {
       Plan q;
-      const int r = make_plan_impl(dims_in, perm1, rank, elem, n_sub, alpha, p2, o, 0, q);
+      const int r =
+          make_plan_impl(dims_in, perm1, rank, elem, n_sub, alpha, p2, o, 0, q, &shared);
       if (r) {
         if (rc) rc = r;
         continue;
@@ -1134,6 +1168,7 @@ int make_plan(const int64_t* dims_in, const int* perm1, int rank, size_t elem, i
       }
       rc = EITHOT_OK;
     }
+  }
   if (rc) return rc;
   DeviceProfile pb = prof;
   pb.staged_mult = best_mult;

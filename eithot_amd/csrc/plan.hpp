@@ -124,6 +124,7 @@ struct Op {
   std::vector<CycleMap> cmaps;            // OP_CYCLE: cell maps (in units of U-byte cells)
   std::shared_ptr<const CycleTable> cyc;  // OP_CYCLE: segments shared by all B regions
   std::shared_ptr<const BlockTable> blk;  // OP_BLOCK: B blocks of R cells of U bytes
+  std::vector<int64_t> bparts;            // OP_BLOCK: the fused tiles, (R, C, U) each
 
   int64_t footprint_units() const;   // units staged per footprint
   int64_t footprint_bytes() const { return footprint_units() * us; }
