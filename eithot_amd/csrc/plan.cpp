@@ -308,6 +308,8 @@ int64_t pow2_factor(int64_t x, int64_t cap) {
   return w;
 }
 
+const std::vector<int64_t>& divisors(int64_t x);  // ascending, memoized
+
 // Largest slice of a U-byte unit such that `units` slices fit `budget`
 // bytes, keeping the unit's power-of-two alignment (so the word width of the
 // copy does not shrink) and at least `min_seg` contiguous bytes.  0 if none.
@@ -317,14 +319,11 @@ int64_t pick_slice(int64_t units, int64_t U, int64_t budget, int64_t min_seg) {
   const int64_t align = pow2_factor(U, 16);
   const int64_t cap = budget / units;
   if (cap < min_seg) return 0;
-  int64_t best = 0;
-  // divisors of U that are multiples of `align`, at most cap
-  for (int64_t q = 1; q * q <= U; ++q) {
-    if (U % q) continue;
-    for (int64_t us : {q, U / q})
-      if (us % align == 0 && us <= cap && us >= min_seg && us > best) best = us;
-  }
-  return best;
+  // the largest divisor of U that is a multiple of `align`, at most cap
+  const std::vector<int64_t>& dv = divisors(U);
+  for (auto it = dv.rbegin(); it != dv.rend(); ++it)
+    if (*it % align == 0 && *it <= cap && *it >= min_seg) return *it;
+  return 0;
 }
 
 // Tile footprint (R rows of C slices, padded rows): whole units if they fit
@@ -335,13 +334,10 @@ int64_t pick_tile_slice(int64_t R, int64_t C, int64_t U, const DeviceProfile& pr
   auto fp = [&](int64_t us) { return R * tile_row_stride_bound(C * us, us); };
   const int64_t align = pow2_factor(U, 16);
   auto largest = [&](int64_t cap, int64_t min_seg) {
-    int64_t best = 0;
-    for (int64_t q = 1; q * q <= U; ++q) {
-      if (U % q) continue;
-      for (int64_t us : {q, U / q})
-        if (us % align == 0 && us >= min_seg && us > best && fp(us) <= cap) best = us;
-    }
-    return best;
+    const std::vector<int64_t>& dv = divisors(U);
+    for (auto it = dv.rbegin(); it != dv.rend() && *it >= min_seg; ++it)
+      if (*it % align == 0 && fp(*it) <= cap) return *it;
+    return (int64_t)0;
   };
   const int64_t target = prof.tile_fp_target > 0 ? std::min(prof.tile_fp_target, prof.lds_max)
                                                  : prof.lds_max;
@@ -366,13 +362,10 @@ int64_t pick_staged_tile_slice(int64_t R, int64_t C, int64_t U, int64_t budget,
 // Largest divisor of U whose `units` slices fit `cap` bytes, whatever its
 // alignment (narrow slices cost word width, but always exist).  0 if none.
 int64_t any_slice(int64_t units, int64_t U, int64_t cap) {
-  int64_t best = 0;
-  for (int64_t q = 1; q * q <= U; ++q) {
-    if (U % q) continue;
-    for (int64_t us : {q, U / q})
-      if (us > best && units * us <= cap) best = us;
-  }
-  return best;
+  const std::vector<int64_t>& dv = divisors(U);
+  for (auto it = dv.rbegin(); it != dv.rend(); ++it)
+    if (units * *it <= cap) return *it;
+  return 0;
 }
 
 struct CatPlan {
@@ -533,7 +526,7 @@ struct Lowerer {
       ch.cost = 0;
       return ch;
     }
-    auto key = std::make_tuple(R, C, U, depth);
+    auto key = std::make_tuple(R, C, U, 0);  // first result wins at any depth
     auto it = memo.find(key);
     if (it != memo.end()) return it->second;
     if (int64_t us = pick_tile_slice(R, C, U, prof)) {
