@@ -42,7 +42,9 @@ extern "C" {
  * is done.  n_sub > 1 asks for the reference's sub-tensor partitioning along
  * the largest dim (applied when that dim exceeds floor(32/alpha); the count
  * used is the largest divisor of the dim not above n_sub); n_sub <= 0 takes
- * the estimator's choice.  Extra device memory stays <= alpha * tensor bytes.
+ * the estimator's choice (eithot_estimate: 1, since this build's plans meet
+ * alpha without partitioning).  Extra device memory stays <= alpha * tensor
+ * bytes, or one pass footprint when that is larger.
  * Replaces inplace::transpose (ITNT/include/transpose.h:8,
  * ITNT/src/transpose.cpp:21-30). */
 int eithot_transpose(void* dev, const int64_t* dims, const int* perm, int order,
@@ -61,8 +63,9 @@ int eithot_inplace_transpose(void* data, int source, int rank, const int* dim,
                              const int* permutation, size_t sizeofType, int num_block,
                              double large_ratio);
 
-/* Peak extra device bytes (scratch) used by the calling thread's last
- * transpose; 0 when every pass ran through LDS. */
+/* Peak extra device bytes used by the calling thread's last transpose:
+ * scratch plus the tables of its cycle-following and fused passes; 0 when
+ * every pass ran through LDS without tables. */
 size_t eithot_last_peak_extra_bytes(void);
 
 /* Message of the calling thread's last failure ("" if none). */
