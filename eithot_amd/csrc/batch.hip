@@ -83,7 +83,8 @@ struct eithot_batch {
       for (size_t t = 0; t < plans.size(); ++t)
         if (k < plans[t].ops.size()) ops.push_back({&plans[t].ops[k], tensors[t].dev});
       std::vector<size_t> rejected;
-      int rc = batch_launch_build(ops, c, gpc, dev->prof.cus, &out[k].persistent, &rejected);
+      int rc = batch_launch_build(ops, c, gpc, dev->prof.cus, scratch, scratch_bytes,
+                                  &out[k].persistent, &rejected);
       if (rc) return rc;
       for (size_t i : rejected) out[k].side.push_back(ops[i]);
     }
@@ -159,6 +160,19 @@ int eithot_batch_build(eithot_batch* b, double alpha, int tune) {
                      b->inv[t]);
       if (rc) return fail(rc, "planning inverse of tensor %zu failed (%d)", t, rc);
       b->scratch_bytes = std::max(b->scratch_bytes, b->inv[t].scratch_bytes);
+    }
+  }
+  // the batched cycle passes of one stage run together: each keeps its own
+  // boundary cells, so a stage needs the sum of theirs (side passes run
+  // after the batched launch and reuse the buffer from its start)
+  for (const std::vector<Plan>* plans : {&b->fwd, &b->inv}) {
+    size_t depth = 0;
+    for (const Plan& p : *plans) depth = std::max(depth, p.ops.size());
+    for (size_t k = 0; k < depth; ++k) {
+      int64_t sum = 0;
+      for (const Plan& p : *plans)
+        if (k < p.ops.size()) sum += batch_boundary_bytes(p.ops[k]);
+      b->scratch_bytes = std::max(b->scratch_bytes, sum);
     }
   }
   if (b->scratch_bytes > 0 && hipMalloc((void**)&b->scratch, (size_t)b->scratch_bytes) != hipSuccess) {
@@ -250,7 +264,7 @@ int eithot_batch_info(const eithot_batch* b, int* launches, int64_t* scratch_byt
                       int* graph_captured) {
   if (!b) return fail(EITHOT_ERR_INVALID, "null batch");
   int n = 0;
-  for (const auto& s : b->stages) n += (s.persistent ? 1 : 0) + (int)s.side.size();
+  for (const auto& s : b->stages) n += batch_launch_count(s.persistent) + (int)s.side.size();
   if (launches) *launches = n;
   if (scratch_bytes) *scratch_bytes = b->scratch_bytes;
   if (graph_captured) *graph_captured = b->exec != nullptr;

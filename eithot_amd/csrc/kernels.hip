@@ -17,6 +17,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <type_traits>
 
 #include "eithot_status.h"
@@ -938,60 +939,6 @@ hipError_t run_op(const Op& op, char* d, char* s, int64_t sb, const LaunchCfg& c
 // persistent batched pass: one launch runs one pass of many tensors; each
 // workgroup walks a table of (op, first footprint) items
 // ---------------------------------------------------------------------------
-struct BatchOpDev {
-  int variant;
-  int pad[3];
-  Args<uint32_t> a;
-};
-struct BatchItemDev {
-  uint32_t op;
-  uint32_t f0;
-};
-
-template <int KM, int WL, int VF>
-__device__ __forceinline__ void batch_body(const Args<uint32_t>& a, uint32_t f0, unsigned char* lds) {
-  using Word = typename std::conditional<
-      WL == 4, W16,
-      typename std::conditional<
-          WL == 3, uint64_t,
-          typename std::conditional<WL == 2, uint32_t,
-                                    typename std::conditional<WL == 1, uint16_t, uint8_t>::type>::
-              type>::type>::type;
-  constexpr int V = VF ? (16 >> WL) : 1;
-  if (KM == 0) tile_body<Word, uint32_t, V>(a, f0, lds);
-  else if (KM <= 4) col_body<Word, uint32_t, (KM <= 4 ? KM : 1), V>(a, f0, lds);
-  else row_body<Word, uint32_t, (KM >= 5 ? KM : 5), V>(a, f0, lds);
-}
-
-#define EH_CASE(KM, WL, VF) \
-  case KM * 10 + WL * 2 + VF: batch_body<KM, WL, VF>(op.a, item.f0, lds); break;
-#define EH_CASES_W(KM, WL) EH_CASE(KM, WL, 0) EH_CASE(KM, WL, 1)
-#define EH_CASES(KM) \
-  EH_CASES_W(KM, 0) EH_CASES_W(KM, 1) EH_CASES_W(KM, 2) EH_CASES_W(KM, 3) EH_CASE(KM, 4, 0)
-
-__global__ __launch_bounds__(512) void k_batch(const BatchOpDev* __restrict__ ops,
-                                               const BatchItemDev* __restrict__ items,
-                                               uint32_t nitems) {
-  extern __shared__ __align__(16) unsigned char lds[];
-  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const BatchItemDev item = items[it];
-    const BatchOpDev& op = ops[item.op];
-    switch (op.variant) {
-      EH_CASES(0)
-      EH_CASES(1)
-      EH_CASES(2)
-      EH_CASES(3)
-      EH_CASES(4)
-      EH_CASES(5)
-      EH_CASES(6)
-      default: break;
-    }
-    __syncthreads();  // the next item reuses the LDS
-  }
-}
-#undef EH_CASES
-#undef EH_CASES_W
-#undef EH_CASE
 
 // ---- pattern fill / verify ----------------------------------------------------
 
@@ -1156,12 +1103,12 @@ __device__ __forceinline__ void st_nt(char* p, const Word& w) {
   }
 }
 
-template <typename Word, int D, int NT = 0>
-__global__ __launch_bounds__(256) void k_cycle(CycArgs a) {
+// gtid: the thread's index among all threads of the pass
+template <typename Word, int D, int NT>
+__device__ __forceinline__ void cycle_body(const CycArgs& a, const uint64_t gtid) {
   constexpr uint64_t w = sizeof(Word);
   // an item is 2^lane_log2 consecutive lanes: a whole wave, or a group of
   // lanes when cells are smaller than 64 words (several items per wave)
-  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t lane = (uint32_t)(gtid & ((1u << a.lane_log2) - 1));
   const uint64_t item = gtid >> a.lane_log2;
   if (item >= a.items) return;
@@ -1198,6 +1145,26 @@ __global__ __launch_bounds__(256) void k_cycle(CycArgs a) {
       }
     p = q[D - 1];
   }
+}
+
+template <typename Word, int D, int NT = 0>
+__global__ __launch_bounds__(256) void k_cycle(CycArgs a) {
+  cycle_body<Word, D, NT>(a, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Boundary save of a cycle pass, one word per thread: gtid indexes the
+// words of all B * nslots saved cells.
+template <typename Word>
+__device__ __forceinline__ void cycle_save_word(const CycArgs& a, const uint64_t t) {
+  constexpr uint64_t w = sizeof(Word);
+  const uint64_t wpc = a.U / w;
+  if (t >= a.B * a.nslots * wpc) return;
+  const uint64_t cell = t / wpc;  // boundary cell = b * nslots + slot
+  const uint64_t x = t - cell * wpc;
+  const uint64_t b = cell / a.nslots;
+  const uint64_t sl = cell - b * a.nslots;
+  *reinterpret_cast<Word*>(a.boundary + cell * a.U + x * w) = *reinterpret_cast<const Word*>(
+      a.data + a.base + b * a.RCU + (uint64_t)a.slots[sl] * a.U + x * w);
 }
 
 // Pipelined variant: every lane moves V words per position (a wave covers
@@ -1334,18 +1301,25 @@ hipError_t launch_cycle_w(CycArgs a, int variant, hipStream_t stream) {
 // lanes' words per position of each cycle kernel variant
 inline int cycle_vwords(int variant) { return variant == 1 ? 2 : variant == 3 ? 4 : 1; }
 
-hipError_t launch_cycle(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
-                        int variant, hipStream_t stream) {
+// Arguments of a cycle pass over the tensor at `data` with its boundary
+// cells in `scratch`; *w receives the word size, *variant the kernel shape
+// it can use.  *empty is set when the pass moves nothing (fixed points only).
+hipError_t prepare_cycle_args(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
+                              int* variant, CycArgs& a, int64_t* w_out, bool* empty) {
+  *empty = false;
   if (!op.cyc || (uint64_t)op.R * (uint64_t)op.C >= (1ull << 32)) return hipErrorInvalidValue;
   const CycleTable& t = *op.cyc;
-  if (t.segs.empty()) return hipSuccess;  // only fixed points
+  if (t.segs.empty()) {
+    *empty = true;
+    return hipSuccess;
+  }
   const uint64_t need = (uint64_t)op.B * t.slots.size() * (uint64_t)op.U;
   if (need && (!scratch || (uint64_t)scratch_bytes < need)) return hipErrorInvalidValue;
   hipError_t e;
   const char* d = (const char*)cycle_table_device(t, &e);
   if (!d) return e;
   const int64_t w = word_bytes(op, (uintptr_t)data);
-  CycArgs a;
+  *w_out = w;
   a.data = data;
   a.base = op.base;
   a.RCU = (uint64_t)op.R * op.C * op.U;
@@ -1370,14 +1344,24 @@ hipError_t launch_cycle(const Op& op, char* data, char* scratch, int64_t scratch
   // cells smaller than a wave's row of words: 2^lane_log2 lanes per cell
   a.lane_log2 = 6;
   while (a.lane_log2 > 0 && ((uint64_t)w << (a.lane_log2 - 1)) >= (uint64_t)op.U) --a.lane_log2;
-  if (a.lane_log2 < 6) variant = 0;
-  a.chunk_bytes = (uint32_t)((w << a.lane_log2) * cycle_vwords(variant));
+  if (a.lane_log2 < 6) *variant = 0;
+  a.chunk_bytes = (uint32_t)((w << a.lane_log2) * cycle_vwords(*variant));
   a.B = op.B;
   a.chunks = (op.U + a.chunk_bytes - 1) / a.chunk_bytes;
   a.items = a.B * a.nseg * a.chunks;
   a.fchunks.init(a.chunks);
   a.fnseg.init(a.nseg);
   if (((a.items << a.lane_log2) + 255) / 256 >= (1ull << 31)) return hipErrorInvalidValue;
+  return hipSuccess;
+}
+
+hipError_t launch_cycle(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
+                        int variant, hipStream_t stream) {
+  CycArgs a;
+  int64_t w = 1;
+  bool empty = false;
+  hipError_t e = prepare_cycle_args(op, data, scratch, scratch_bytes, &variant, a, &w, &empty);
+  if (e != hipSuccess || empty) return e;
   switch (w) {
     case 16: return launch_cycle_w<W16>(a, variant, stream);
     case 8: return launch_cycle_w<uint64_t>(a, variant, stream);
@@ -1426,11 +1410,10 @@ __device__ __forceinline__ uint32_t lds_off(uint32_t x, uint32_t pad) {
   return pad ? x + ((x >> pad) << 2) : x;
 }
 
+// f0: first block of the workgroup's K blocks
 template <typename Word, int V>
-__global__ __launch_bounds__(512) void k_block(BlkArgs a, int remap) {
-  extern __shared__ __align__(16) unsigned char lds[];
+__device__ __forceinline__ void block_body(const BlkArgs& a, const uint64_t f0, unsigned char* lds) {
   constexpr uint32_t w = sizeof(Word);
-  const uint64_t f0 = (uint64_t)xcd_logical(blockIdx.x, gridDim.x, remap) * a.K;
   const uint32_t nf = (uint32_t)((a.B - f0) < a.K ? (a.B - f0) : a.K);
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   const char* g = a.data + a.base + f0 * a.F;
@@ -1502,6 +1485,12 @@ __global__ __launch_bounds__(512) void k_block(BlkArgs a, int remap) {
   }
 }
 
+template <typename Word, int V>
+__global__ __launch_bounds__(512) void k_block(BlkArgs a, int remap) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  block_body<Word, V>(a, (uint64_t)xcd_logical(blockIdx.x, gridDim.x, remap) * a.K, lds);
+}
+
 const void* block_table_device(const BlockTable& t, hipError_t* err) {
   *err = hipSuccess;
   int dev = 0;
@@ -1537,7 +1526,10 @@ hipError_t launch_block_w(BlkArgs a, int vec, int64_t lds, int64_t groups, const
   return hipGetLastError();
 }
 
-hipError_t launch_block(const Op& op, char* data, const LaunchCfg& cfg, hipStream_t stream) {
+// Arguments of a fused block pass; *w: word size, *vec: packed stores,
+// *lds: LDS bytes per workgroup, *groups: workgroups of K blocks.
+hipError_t prepare_block_args(const Op& op, char* data, const LaunchCfg& cfg, BlkArgs& a,
+                              int64_t* w_out, int* vec_out, int64_t* lds_out, int64_t* groups_out) {
   if (!op.blk || op.blk->cells != op.R || op.blk->cell_bytes != op.U) return hipErrorInvalidValue;
   const BlockTable& t = *op.blk;
   const int64_t F = op.R * op.U;
@@ -1548,7 +1540,6 @@ hipError_t launch_block(const Op& op, char* data, const LaunchCfg& cfg, hipStrea
   const uintptr_t base = (uintptr_t)(data + op.base);
   int64_t w = 16;
   while (w > 1 && ((op.U % w) || (base % w) || (F % w))) w /= 2;
-  BlkArgs a;
   a.data = data;
   a.base = op.base;
   a.F = (uint64_t)F;
@@ -1582,7 +1573,19 @@ hipError_t launch_block(const Op& op, char* data, const LaunchCfg& cfg, hipStrea
   }
   const int64_t groups = (op.B + K - 1) / K;
   if (groups >= (1ll << 31)) return hipErrorInvalidValue;
-  const int64_t lds = K * (int64_t)a.Fl;
+  *w_out = w;
+  *vec_out = vec;
+  *lds_out = K * (int64_t)a.Fl;
+  *groups_out = groups;
+  return hipSuccess;
+}
+
+hipError_t launch_block(const Op& op, char* data, const LaunchCfg& cfg, hipStream_t stream) {
+  BlkArgs a;
+  int64_t w = 1, lds = 0, groups = 0;
+  int vec = 0;
+  hipError_t e = prepare_block_args(op, data, cfg, a, &w, &vec, &lds, &groups);
+  if (e != hipSuccess) return e;
   switch (w) {
     case 16: return launch_block_w<W16>(a, vec, lds, groups, cfg, stream);
     case 8: return launch_block_w<uint64_t>(a, vec, lds, groups, cfg, stream);
@@ -1591,6 +1594,109 @@ hipError_t launch_block(const Op& op, char* data, const LaunchCfg& cfg, hipStrea
     default: return launch_block_w<uint8_t>(a, vec, lds, groups, cfg, stream);
   }
 }
+
+// ---------------------------------------------------------------------------
+// persistent batched launch: items of many tensors' passes in one kernel
+// ---------------------------------------------------------------------------
+// An op is the arguments of one tensor's pass; its variant selects the body:
+//   KM*10 + log2(word)*2 + vec  LDS tile/column/row passes (KM 0..6)
+//   70 + log2(word)             cycle following (item: blockDim threads)
+//   80 + log2(word)*2 + vec     fused block permutation (item: K blocks)
+//   90 + log2(word)             boundary save of a cycle pass (item: blockDim words)
+constexpr size_t kBatchArgBytes =
+    sizeof(Args<uint32_t>) > sizeof(CycArgs)
+        ? (sizeof(Args<uint32_t>) > sizeof(BlkArgs) ? sizeof(Args<uint32_t>) : sizeof(BlkArgs))
+        : (sizeof(CycArgs) > sizeof(BlkArgs) ? sizeof(CycArgs) : sizeof(BlkArgs));
+struct BatchOpDev {
+  int variant;
+  int pad[3];
+  alignas(16) unsigned char raw[(kBatchArgBytes + 15) / 16 * 16];
+};
+struct BatchItemDev {
+  uint32_t op;
+  uint32_t f0;
+};
+
+template <int WL>
+struct WordOf {
+  using T = typename std::conditional<
+      WL == 4, W16,
+      typename std::conditional<
+          WL == 3, uint64_t,
+          typename std::conditional<WL == 2, uint32_t,
+                                    typename std::conditional<WL == 1, uint16_t, uint8_t>::type>::
+              type>::type>::type;
+};
+
+template <int KM, int WL, int VF>
+__device__ __forceinline__ void batch_body(const BatchOpDev& op, uint32_t f0, unsigned char* lds) {
+  using Word = typename WordOf<WL>::T;
+  constexpr int V = VF ? (16 >> WL) : 1;
+  const Args<uint32_t>& a = *reinterpret_cast<const Args<uint32_t>*>(op.raw);
+  if (KM == 0) tile_body<Word, uint32_t, V>(a, f0, lds);
+  else if (KM <= 4) col_body<Word, uint32_t, (KM <= 4 ? KM : 1), V>(a, f0, lds);
+  else row_body<Word, uint32_t, (KM >= 5 ? KM : 5), V>(a, f0, lds);
+}
+
+#define EH_CASE(KM, WL, VF) \
+  case KM * 10 + WL * 2 + VF: batch_body<KM, WL, VF>(op, item.f0, lds); break;
+#define EH_CASES_W(KM, WL) EH_CASE(KM, WL, 0) EH_CASE(KM, WL, 1)
+#define EH_CASES(KM) \
+  EH_CASES_W(KM, 0) EH_CASES_W(KM, 1) EH_CASES_W(KM, 2) EH_CASES_W(KM, 3) EH_CASE(KM, 4, 0)
+#define EH_CYC(WL)                                                                           \
+  case 70 + WL:                                                                              \
+    cycle_body<typename WordOf<WL>::T, 8, 0>(*reinterpret_cast<const CycArgs*>(op.raw),      \
+                                             (uint64_t)item.f0 * blockDim.x + threadIdx.x); \
+    break;                                                                                   \
+  case 90 + WL:                                                                              \
+    cycle_save_word<typename WordOf<WL>::T>(*reinterpret_cast<const CycArgs*>(op.raw),       \
+                                            (uint64_t)item.f0 * blockDim.x + threadIdx.x);   \
+    break;
+#define EH_BLK(WL, VF)                                                                      \
+  case 80 + WL * 2 + VF:                                                                    \
+    block_body<typename WordOf<WL>::T, VF ? (16 >> WL) : 1>(                                \
+        *reinterpret_cast<const BlkArgs*>(op.raw), item.f0, lds);                           \
+    break;
+
+__global__ __launch_bounds__(512) void k_batch(const BatchOpDev* __restrict__ ops,
+                                               const BatchItemDev* __restrict__ items,
+                                               uint32_t nitems) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const BatchItemDev item = items[it];
+    const BatchOpDev& op = ops[item.op];
+    switch (op.variant) {
+      EH_CASES(0)
+      EH_CASES(1)
+      EH_CASES(2)
+      EH_CASES(3)
+      EH_CASES(4)
+      EH_CASES(5)
+      EH_CASES(6)
+      EH_CYC(0)
+      EH_CYC(1)
+      EH_CYC(2)
+      EH_CYC(3)
+      EH_CYC(4)
+      EH_BLK(0, 0)
+      EH_BLK(0, 1)
+      EH_BLK(1, 0)
+      EH_BLK(1, 1)
+      EH_BLK(2, 0)
+      EH_BLK(2, 1)
+      EH_BLK(3, 0)
+      EH_BLK(3, 1)
+      EH_BLK(4, 0)
+      default: break;
+    }
+    __syncthreads();  // the next item reuses the LDS
+  }
+}
+#undef EH_BLK
+#undef EH_CYC
+#undef EH_CASES
+#undef EH_CASES_W
+#undef EH_CASE
 
 }  // namespace
 
@@ -1617,47 +1723,135 @@ hipError_t launch_op(const Op& op, char* data, char* scratch, int64_t scratch_by
 // ---------------------------------------------------------------------------
 struct BatchLaunch {
   BatchOpDev* d_ops = nullptr;
-  BatchItemDev* d_items = nullptr;
-  uint32_t nitems = 0;
+  BatchItemDev* d_items = nullptr;  // boundary-save items first, then the passes' items
+  uint32_t nsave = 0;               // items of the save launch
+  uint32_t nitems = 0;              // items of the main launch
   uint32_t grid = 0;
   int threads = 256;
   int64_t lds = 0;
 };
 
+static int log2i(int64_t w) {
+  int l = 0;
+  while ((1ll << l) < w) ++l;
+  return l;
+}
+
+int64_t batch_boundary_bytes(const Op& op) {
+  if (op.kind != OP_CYCLE || !op.cyc) return 0;
+  return ((int64_t)op.B * (int64_t)op.cyc->slots.size() * op.U + 255) / 256 * 256;
+}
+
 int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, const LaunchCfg& cfg,
-                       int grid_per_cu, int cus, BatchLaunch** out, std::vector<size_t>* rejected) {
+                       int grid_per_cu, int cus, char* scratch, int64_t scratch_bytes,
+                       BatchLaunch** out, std::vector<size_t>* rejected) {
   *out = nullptr;
+  // workgroup size: the larger one when any LDS footprint needs it
+  bool big = false;
+  for (const auto& pr : ops) {
+    const Op& op = *pr.first;
+    if (op.kind == OP_BLOCK) {
+      BlkArgs a;
+      int64_t w, lds, groups;
+      int vec;
+      if (prepare_block_args(op, pr.second, cfg, a, &w, &vec, &lds, &groups) == hipSuccess)
+        big = big || lds > 96 * 1024;
+    } else if (!op.staged && op.kind >= OP_TILE && op.kind <= OP_ROW && !needs64(op)) {
+      Args<uint32_t> a;
+      Prepared p;
+      prepare_lds<uint32_t>(op, pr.second, cfg, a, p);
+      big = big || p.threads > cfg.threads;
+    }
+  }
+  const int threads = big ? cfg.threads_big : cfg.threads;
   std::vector<BatchOpDev> hops;
   std::vector<std::pair<int64_t, BatchItemDev>> items;
-  int64_t lds = 0;
-  bool big = false;
+  std::vector<BatchItemDev> saves;
+  int64_t lds = 0, boff = 0;
+  auto reject = [&](size_t i) {
+    if (rejected) rejected->push_back(i);
+  };
   for (size_t i = 0; i < ops.size(); ++i) {
     const Op& op = *ops[i].first;
-    if (op.staged || op.kind == 0 || op.kind == OP_CYCLE || op.kind == OP_BLOCK || needs64(op)) {
-      if (rejected) rejected->push_back(i);
+    char* data = ops[i].second;
+    BatchOpDev d{};
+    if (op.kind == OP_CYCLE) {
+      // each tensor's boundary cells get their own part of the scratch
+      const int64_t need = batch_boundary_bytes(op);
+      if (boff + need > scratch_bytes) {
+        reject(i);
+        continue;
+      }
+      CycArgs a;
+      int64_t w = 1;
+      int variant = 0;
+      bool empty = false;
+      if (prepare_cycle_args(op, data, need ? scratch + boff : nullptr, need, &variant, a, &w,
+                             &empty) != hipSuccess) {
+        reject(i);
+        continue;
+      }
+      if (empty) continue;
+      boff += need;
+      std::memcpy(d.raw, &a, sizeof(a));
+      d.variant = 70 + log2i(w);
+      const uint32_t idx = (uint32_t)hops.size();
+      hops.push_back(d);
+      const uint64_t groups = ((a.items << a.lane_log2) + threads - 1) / threads;
+      for (uint64_t g = 0; g < groups; ++g) items.push_back({0, BatchItemDev{idx, (uint32_t)g}});
+      if (a.nslots) {
+        d.variant = 90 + log2i(w);
+        const uint32_t sidx = (uint32_t)hops.size();
+        hops.push_back(d);
+        const uint64_t words = a.B * a.nslots * (a.U / (uint64_t)w);
+        for (uint64_t g = 0; g < (words + threads - 1) / threads; ++g)
+          saves.push_back(BatchItemDev{sidx, (uint32_t)g});
+      }
       continue;
     }
-    BatchOpDev d{};
+    if (op.kind == OP_BLOCK) {
+      BlkArgs a;
+      int64_t w = 1, blds = 0, groups = 0;
+      int vec = 0;
+      if (prepare_block_args(op, data, cfg, a, &w, &vec, &blds, &groups) != hipSuccess) {
+        reject(i);
+        continue;
+      }
+      std::memcpy(d.raw, &a, sizeof(a));
+      d.variant = 80 + log2i(w) * 2 + (vec && w < 16 ? 1 : 0);
+      const uint32_t idx = (uint32_t)hops.size();
+      hops.push_back(d);
+      lds = std::max(lds, blds);
+      for (int64_t g = 0; g < groups; ++g)
+        items.push_back({blds, BatchItemDev{idx, (uint32_t)(g * a.K)}});
+      continue;
+    }
+    if (op.staged || op.kind < OP_TILE || op.kind > OP_ROW || needs64(op)) {
+      reject(i);
+      continue;
+    }
+    Args<uint32_t> a;
     Prepared p;
-    prepare_lds<uint32_t>(op, ops[i].second, cfg, d.a, p);
+    prepare_lds<uint32_t>(op, data, cfg, a, p);
+    std::memcpy(d.raw, &a, sizeof(a));
     d.variant = p.variant;
     const uint32_t idx = (uint32_t)hops.size();
     hops.push_back(d);
     lds = std::max(lds, p.lds);
-    big = big || p.threads > cfg.threads;
     for (int64_t g = 0; g < p.groups; ++g)
       items.push_back({p.lds, BatchItemDev{idx, (uint32_t)(g * p.K)}});
   }
-  if (items.empty()) return EITHOT_OK;
+  if (items.empty() && saves.empty()) return EITHOT_OK;
   // largest items first so that round-robin assignment balances the tail
   std::stable_sort(items.begin(), items.end(),
                    [](const std::pair<int64_t, BatchItemDev>& x,
                       const std::pair<int64_t, BatchItemDev>& y) { return x.first > y.first; });
-  std::vector<BatchItemDev> hitems(items.size());
-  for (size_t i = 0; i < items.size(); ++i) hitems[i] = items[i].second;
+  std::vector<BatchItemDev> hitems(saves);
+  for (const auto& it : items) hitems.push_back(it.second);
   BatchLaunch* L = new BatchLaunch;
-  L->nitems = (uint32_t)hitems.size();
-  L->threads = big ? cfg.threads_big : cfg.threads;
+  L->nsave = (uint32_t)saves.size();
+  L->nitems = (uint32_t)items.size();
+  L->threads = threads;
   L->lds = lds;
   L->grid = grid_per_cu > 0 ? (uint32_t)std::min<int64_t>(L->nitems, (int64_t)cus * grid_per_cu)
                             : L->nitems;
@@ -1678,9 +1872,15 @@ int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, cons
 }
 
 hipError_t batch_launch_enqueue(const BatchLaunch* L, hipStream_t st) {
-  if (!L || !L->nitems) return hipSuccess;
-  hipLaunchKernelGGL(k_batch, dim3(L->grid), dim3(L->threads), (size_t)L->lds, st, L->d_ops,
-                     L->d_items, L->nitems);
+  if (!L) return hipSuccess;
+  if (L->nsave) {  // boundary cells of every cycle pass, before any of them writes
+    const uint32_t grid = std::min<uint32_t>(L->nsave, 256u * 16u);
+    hipLaunchKernelGGL(k_batch, dim3(grid), dim3(L->threads), 0, st, L->d_ops, L->d_items,
+                       L->nsave);
+  }
+  if (L->nitems)
+    hipLaunchKernelGGL(k_batch, dim3(L->grid ? L->grid : 1), dim3(L->threads), (size_t)L->lds, st,
+                       L->d_ops, L->d_items + L->nsave, L->nitems);
   return hipGetLastError();
 }
 
@@ -1691,7 +1891,11 @@ void batch_launch_free(BatchLaunch* L) {
   delete L;
 }
 
-int64_t batch_launch_items(const BatchLaunch* L) { return L ? L->nitems : 0; }
+int64_t batch_launch_items(const BatchLaunch* L) { return L ? L->nitems + L->nsave : 0; }
+
+int batch_launch_count(const BatchLaunch* L) {
+  return L ? (L->nsave ? 1 : 0) + (L->nitems ? 1 : 0) : 0;
+}
 
 hipError_t launch_fill(char* data, uint64_t n, int elem, uint64_t seed, int mode, uint64_t first,
                        hipStream_t stream) {

@@ -45,15 +45,19 @@ hipError_t launch_verify(const char* data, const int64_t* dims, const int* perm0
 
 // Persistent batched launch: one pass (op) of many tensors in one kernel.
 // ops pairs each op with its tensor's device pointer; staged or 64-bit-index
-// ops are not batched and are returned in `rejected` (indices into ops).
-// grid_per_cu > 0 sizes a persistent grid (workgroups per CU), 0 launches
-// one workgroup per item.
+// ops are not batched and are returned in `rejected` (indices into ops), as
+// are cycle passes whose boundary cells no longer fit `scratch` (each cycle
+// pass gets batch_boundary_bytes of it).  grid_per_cu > 0 sizes a persistent
+// grid (workgroups per CU), 0 launches one workgroup per item.
 struct BatchLaunch;
+int64_t batch_boundary_bytes(const Op& op);
 int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, const LaunchCfg& cfg,
-                       int grid_per_cu, int cus, BatchLaunch** out, std::vector<size_t>* rejected);
+                       int grid_per_cu, int cus, char* scratch, int64_t scratch_bytes,
+                       BatchLaunch** out, std::vector<size_t>* rejected);
 hipError_t batch_launch_enqueue(const BatchLaunch* L, hipStream_t stream);
 void batch_launch_free(BatchLaunch* L);
 int64_t batch_launch_items(const BatchLaunch* L);
+int batch_launch_count(const BatchLaunch* L);  // kernel launches per enqueue (0..2)
 
 // Raise the dynamic-LDS limit of every pass kernel to `bytes` (once per device).
 hipError_t configure_kernels(int64_t bytes);
