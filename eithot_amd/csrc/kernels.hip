@@ -1722,13 +1722,17 @@ hipError_t launch_op(const Op& op, char* data, char* scratch, int64_t scratch_by
 // persistent batched launches (host side)
 // ---------------------------------------------------------------------------
 struct BatchLaunch {
+  // one k_batch launch per part, in order: boundary saves, cycle items (no
+  // LDS), then LDS items in classes of footprint size so that a few large
+  // footprints do not hold every workgroup of the stage to one per CU
+  struct Part {
+    uint32_t offset, count, grid;
+    int64_t lds;
+  };
   BatchOpDev* d_ops = nullptr;
-  BatchItemDev* d_items = nullptr;  // boundary-save items first, then the passes' items
-  uint32_t nsave = 0;               // items of the save launch
-  uint32_t nitems = 0;              // items of the main launch
-  uint32_t grid = 0;
+  BatchItemDev* d_items = nullptr;
+  std::vector<Part> parts;
   int threads = 256;
-  int64_t lds = 0;
 };
 
 static int log2i(int64_t w) {
@@ -1766,7 +1770,7 @@ int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, cons
   const int threads = big ? cfg.threads_big : cfg.threads;
   std::vector<BatchOpDev> hops;
   std::vector<std::pair<int64_t, BatchItemDev>> items;
-  std::vector<BatchItemDev> saves;
+  std::vector<BatchItemDev> saves, cycles;  // cycle items run in a launch without LDS
   int64_t lds = 0, boff = 0;
   auto reject = [&](size_t i) {
     if (rejected) rejected->push_back(i);
@@ -1798,7 +1802,7 @@ int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, cons
       const uint32_t idx = (uint32_t)hops.size();
       hops.push_back(d);
       const uint64_t groups = ((a.items << a.lane_log2) + threads - 1) / threads;
-      for (uint64_t g = 0; g < groups; ++g) items.push_back({0, BatchItemDev{idx, (uint32_t)g}});
+      for (uint64_t g = 0; g < groups; ++g) cycles.push_back(BatchItemDev{idx, (uint32_t)g});
       if (a.nslots) {
         d.variant = 90 + log2i(w);
         const uint32_t sidx = (uint32_t)hops.size();
@@ -1841,20 +1845,40 @@ int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, cons
     for (int64_t g = 0; g < p.groups; ++g)
       items.push_back({p.lds, BatchItemDev{idx, (uint32_t)(g * p.K)}});
   }
-  if (items.empty() && saves.empty()) return EITHOT_OK;
+  if (items.empty() && saves.empty() && cycles.empty()) return EITHOT_OK;
   // largest items first so that round-robin assignment balances the tail
   std::stable_sort(items.begin(), items.end(),
                    [](const std::pair<int64_t, BatchItemDev>& x,
                       const std::pair<int64_t, BatchItemDev>& y) { return x.first > y.first; });
-  std::vector<BatchItemDev> hitems(saves);
-  for (const auto& it : items) hitems.push_back(it.second);
+  (void)lds;
   BatchLaunch* L = new BatchLaunch;
-  L->nsave = (uint32_t)saves.size();
-  L->nitems = (uint32_t)items.size();
   L->threads = threads;
-  L->lds = lds;
-  L->grid = grid_per_cu > 0 ? (uint32_t)std::min<int64_t>(L->nitems, (int64_t)cus * grid_per_cu)
-                            : L->nitems;
+  std::vector<BatchItemDev> hitems;
+  auto add_part = [&](const std::vector<BatchItemDev>& v, int64_t plds, int64_t per_cu) {
+    if (v.empty()) return;
+    const uint32_t n = (uint32_t)v.size();
+    const uint32_t grid =
+        grid_per_cu > 0 ? (uint32_t)std::min<int64_t>(n, (int64_t)cus * per_cu) : n;
+    L->parts.push_back({(uint32_t)hitems.size(), n, grid, plds});
+    hitems.insert(hitems.end(), v.begin(), v.end());
+  };
+  // without LDS, up to 2048 threads per CU stay resident
+  add_part(saves, 0, 2048 / threads);
+  add_part(cycles, 0, 2048 / threads);
+  // LDS items (sorted by footprint, largest first) in classes of at most
+  // 40 KiB, 80 KiB and the rest
+  for (int64_t cap : {(int64_t)1 << 40, (int64_t)80 * 1024, (int64_t)40 * 1024}) {
+    std::vector<BatchItemDev> cls;
+    int64_t clds = 0;
+    for (const auto& it : items) {
+      const int64_t lo = cap == (int64_t)1 << 40 ? 80 * 1024 : cap == 80 * 1024 ? 40 * 1024 : -1;
+      if (it.first <= cap && it.first > lo) {
+        cls.push_back(it.second);
+        clds = std::max(clds, it.first);
+      }
+    }
+    add_part(cls, clds, grid_per_cu);
+  }
   if (hipMalloc((void**)&L->d_ops, hops.size() * sizeof(BatchOpDev)) != hipSuccess ||
       hipMalloc((void**)&L->d_items, hitems.size() * sizeof(BatchItemDev)) != hipSuccess) {
     batch_launch_free(L);
@@ -1873,14 +1897,11 @@ int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, cons
 
 hipError_t batch_launch_enqueue(const BatchLaunch* L, hipStream_t st) {
   if (!L) return hipSuccess;
-  if (L->nsave) {  // boundary cells of every cycle pass, before any of them writes
-    const uint32_t grid = std::min<uint32_t>(L->nsave, 256u * 16u);
-    hipLaunchKernelGGL(k_batch, dim3(grid), dim3(L->threads), 0, st, L->d_ops, L->d_items,
-                       L->nsave);
-  }
-  if (L->nitems)
-    hipLaunchKernelGGL(k_batch, dim3(L->grid ? L->grid : 1), dim3(L->threads), (size_t)L->lds, st,
-                       L->d_ops, L->d_items + L->nsave, L->nitems);
+  // the boundary-save part comes first: every cycle pass's cells are saved
+  // before any of them writes
+  for (const BatchLaunch::Part& p : L->parts)
+    hipLaunchKernelGGL(k_batch, dim3(p.grid ? p.grid : 1), dim3(L->threads), (size_t)p.lds, st,
+                       L->d_ops, L->d_items + p.offset, p.count);
   return hipGetLastError();
 }
 
@@ -1891,11 +1912,14 @@ void batch_launch_free(BatchLaunch* L) {
   delete L;
 }
 
-int64_t batch_launch_items(const BatchLaunch* L) { return L ? L->nitems + L->nsave : 0; }
-
-int batch_launch_count(const BatchLaunch* L) {
-  return L ? (L->nsave ? 1 : 0) + (L->nitems ? 1 : 0) : 0;
+int64_t batch_launch_items(const BatchLaunch* L) {
+  int64_t n = 0;
+  if (L)
+    for (const BatchLaunch::Part& p : L->parts) n += p.count;
+  return n;
 }
+
+int batch_launch_count(const BatchLaunch* L) { return L ? (int)L->parts.size() : 0; }
 
 hipError_t launch_fill(char* data, uint64_t n, int elem, uint64_t seed, int mode, uint64_t first,
                        hipStream_t stream) {

@@ -57,7 +57,7 @@ int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, cons
 hipError_t batch_launch_enqueue(const BatchLaunch* L, hipStream_t stream);
 void batch_launch_free(BatchLaunch* L);
 int64_t batch_launch_items(const BatchLaunch* L);
-int batch_launch_count(const BatchLaunch* L);  // kernel launches per enqueue (0..2)
+int batch_launch_count(const BatchLaunch* L);  // kernel launches per enqueue
 
 // Raise the dynamic-LDS limit of every pass kernel to `bytes` (once per device).
 hipError_t configure_kernels(int64_t bytes);
