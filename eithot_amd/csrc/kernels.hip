@@ -1386,6 +1386,8 @@ struct BlkArgs {
   FD32 fnw, fwpc, fch, fF16, fFw;             // n * wpc, wpc, chunks, F / 16, F / w
   uint32_t ndig, stride[8];                   // digit form of the table (ndig = 0: table)
   FD32 frad[8];
+  uint32_t dense;  // 1: the K blocks lie in LDS back to back (Fl == F, no padding)
+  FD32 fn;         // n
 };
 
 // source cell of output cell o: digit form, or the gather table
@@ -1417,9 +1419,12 @@ __device__ __forceinline__ void block_body(const BlkArgs& a, const uint64_t f0, 
   const uint32_t nf = (uint32_t)((a.B - f0) < a.K ? (a.B - f0) : a.K);
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   const char* g = a.data + a.base + f0 * a.F;
+  // the workgroup's nf blocks are contiguous: load them as one run of
+  // 16-byte chunks when its length allows (blocks of any size: a chunk may
+  // straddle two blocks when the LDS copy is dense)
   const uint32_t c16 = (uint32_t)(a.F / 16);
-  if (a.vec16) {
-    const uint32_t n16 = nf * c16;
+  if (a.vec16 && ((uint64_t)nf * a.F) % 16 == 0 && ((uintptr_t)g & 15) == 0) {
+    const uint32_t n16 = (uint32_t)((uint64_t)nf * a.F / 16);
     for (uint32_t t0 = tid; t0 < n16; t0 += nt * UNR) {
       W16 v[UNR];
 #pragma unroll
@@ -1431,8 +1436,12 @@ __device__ __forceinline__ void block_body(const BlkArgs& a, const uint64_t f0, 
       for (int u = 0; u < UNR; ++u) {
         const uint32_t t = t0 + u * nt;
         if (t < n16) {
-          const uint32_t k = a.fF16.div(t);
-          lds_put16(lds + k * a.Fl + lds_off((t - k * c16) * 16, a.pad), v[u]);
+          if (a.dense) {
+            *reinterpret_cast<W16*>(lds + t * 16) = v[u];
+          } else {
+            const uint32_t k = a.fF16.div(t);
+            lds_put16(lds + k * a.Fl + lds_off((t - k * c16) * 16, a.pad), v[u]);
+          }
         }
       }
     }
@@ -1447,27 +1456,31 @@ __device__ __forceinline__ void block_body(const BlkArgs& a, const uint64_t f0, 
   }
   __syncthreads();
   char* out = a.data + a.base + f0 * a.F;
-  if (V > 1) {
-    // one word per cell, V cells per 16-byte store
-    const uint32_t total = nf * a.chunks;
+  if (V > 1 && (nf * a.n) % V == 0 && ((uintptr_t)out & 15) == 0) {
+    // one word per cell, V consecutive output cells of the run per 16-byte
+    // store (they may belong to two blocks when n is not a multiple of V)
+    const uint32_t total = nf * a.n / V;
+    const bool inner = a.ndig && a.frad[0].d % V == 0 && a.n % V == 0;
     for (uint32_t t = tid; t < total; t += nt) {
-      const uint32_t k = a.fch.div(t);
-      const uint32_t o0 = (t - k * a.chunks) * V;
+      const uint32_t o0 = t * V;
       Word tmp[V];
-      if (a.ndig && a.frad[0].d % V == 0) {
-        // the V cells differ in the fastest output digit only
-        const uint32_t s0 = blk_src(a, o0);
+      if (inner) {
+        // the V cells lie in one block and differ in its fastest output digit
+        const uint32_t k = a.fn.div(o0);
+        const uint32_t s0 = blk_src(a, o0 - k * a.n);
 #pragma unroll
         for (int v = 0; v < V; ++v)
           tmp[v] = *reinterpret_cast<const Word*>(
               lds + k * a.Fl + lds_off((s0 + v * a.stride[0]) * w, a.pad));
       } else {
 #pragma unroll
-        for (int v = 0; v < V; ++v)
+        for (int v = 0; v < V; ++v) {
+          const uint32_t o = o0 + v, k = a.fn.div(o);
           tmp[v] = *reinterpret_cast<const Word*>(lds + k * a.Fl +
-                                                  lds_off(blk_src(a, o0 + v) * w, a.pad));
+                                                  lds_off(blk_src(a, o - k * a.n) * w, a.pad));
+        }
       }
-      *reinterpret_cast<Vec<V * sizeof(Word)>*>(out + k * a.F + (uint64_t)o0 * w) =
+      *reinterpret_cast<Vec<V * sizeof(Word)>*>(out + (uint64_t)o0 * w) =
           *reinterpret_cast<const Vec<V * sizeof(Word)>*>(tmp);
     }
   } else {
@@ -1547,25 +1560,33 @@ hipError_t prepare_block_args(const Op& op, char* data, const LaunchCfg& cfg, Bl
   a.table = d;
   a.n = (uint32_t)op.R;
   a.wpc = (uint32_t)(op.U / w);
-  a.vec16 = (F % 16 == 0 && base % 16 == 0) ? 1 : 0;
+  a.vec16 = base % 16 == 0 ? 1 : 0;
   a.idx16 = t.cells <= 65536 ? 1 : 0;
   const int64_t V = w < 16 ? 16 / w : 1;
-  const int vec = V > 1 && a.wpc == 1 && op.R % V == 0 && a.vec16;
-  a.chunks = vec ? (uint32_t)(op.R / V) : (uint32_t)op.R;
-  // padding keeps cells of up to 16 bytes whole (pad intervals are >= 64 bytes)
-  // (and LDS words stay aligned: only 1-, 2- and 4-byte words are padded)
-  a.pad = (t.pad_log2 && 64 % op.U == 0 && w <= 4) ? (uint32_t)t.pad_log2 : 0;
+  const int vec = V > 1 && a.wpc == 1 && a.vec16;
+  a.chunks = (uint32_t)op.R;
+  // blocks whose size is not a multiple of 16 bytes lie in LDS back to back
+  // (16-byte loads may then straddle two blocks); otherwise padding keeps
+  // cells of up to 16 bytes whole (pad intervals are >= 64 bytes) and LDS
+  // words aligned (only 1-, 2- and 4-byte words are padded)
+  a.dense = F % 16 ? 1 : 0;
+  a.pad = (!a.dense && t.pad_log2 && 64 % op.U == 0 && w <= 4) ? (uint32_t)t.pad_log2 : 0;
   const int64_t Fl = a.pad ? F + (F >> a.pad) * 4 + 4 : F;
-  a.Fl = (uint32_t)((Fl + 15) / 16 * 16);
+  a.Fl = (uint32_t)(a.dense ? F : (Fl + 15) / 16 * 16);
   if (a.Fl > 160 * 1024) return hipErrorInvalidValue;
   int64_t K = pick_K(op.B, a.Fl, cfg.lds_target);
   while (K > 1 && K * a.Fl > 160 * 1024) --K;
+  // a multiple of the blocks that make 16-byte runs and whole 16-byte stores
+  int64_t mK = 16 / gcd64(F, 16);
+  if (vec) mK = mK / gcd64(mK, V / gcd64(op.R, V)) * (V / gcd64(op.R, V));
+  if (K >= mK) K -= K % mK;
   a.K = (uint32_t)K;
   a.fnw.init(a.n * a.wpc);
   a.fwpc.init(a.wpc);
   a.fch.init(a.chunks);
   a.fF16.init((uint32_t)std::max<int64_t>(F / 16, 1));
   a.fFw.init((uint32_t)(F / w));
+  a.fn.init(a.n);
   a.ndig = (uint32_t)t.ndig;
   for (int k = 0; k < 8; ++k) {
     a.frad[k].init(k < t.ndig ? t.radix[k] : 1);

@@ -104,7 +104,12 @@ def test_fused_block_passes(torch_cuda):
     cases = [([12, 9, 21, 15, 17], [2, 5, 4, 1, 3], 2), ([48, 40, 8, 300], [3, 2, 1, 4], 4),
              ([6, 5, 7, 9, 11], [2, 4, 3, 1, 5], 8), ([10, 13, 18, 12, 23], [5, 3, 4, 2, 1], 1),
              ([2, 6, 20, 3], [1, 4, 3, 2], 8), ([13, 15, 10, 14, 10, 14], [4, 2, 3, 1, 5, 6], 1),
-             ([32, 48, 64, 5, 6, 8], [6, 4, 2, 5, 1, 3], 4)]
+             ([32, 48, 64, 5, 6, 8], [6, 4, 2, 5, 1, 3], 4),
+             # lone tiles of small cells as block passes: blocks of 126 and 8
+             # bytes (not multiples of 16: dense LDS, 16-byte runs and stores
+             # across blocks), a partial last workgroup, 2- and 3-byte cells
+             ([63, 2, 1001, 3], [2, 1, 3, 4], 1), ([4, 2, 30011], [2, 1, 3], 1),
+             ([5, 3, 7, 2001], [2, 1, 3, 4], 2), ([3, 9, 7, 1001], [1, 3, 2, 4], 1)]
     default = E.get_tuning("fuse_max_block")
     try:
         for fuse, digits in ((default, 1), (default, 0), (0, 1)):

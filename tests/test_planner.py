@@ -155,12 +155,12 @@ def test_fused_block_plans_execute_correctly():
     assert sorted(blk[0]["table"].tolist()) == list(range(48 * 40 * 8))
 
 
-COMPOSITE_CASES = [([12, 2, 25, 14, 53, 3], [3, 6, 4, 1, 5, 2], 8),
-                   ([2, 19, 45, 5, 3, 11], [5, 4, 2, 6, 3, 1], 8),
+COMPOSITE_CASES = [([3, 14, 2, 15, 58, 20], [5, 2, 6, 1, 3, 4], 4),
+                   ([12, 2, 25, 14, 53, 3], [3, 6, 4, 1, 5, 2], 8),
                    ([12, 9, 14, 28, 53], [2, 1, 5, 4, 3], 8),
-                   ([2, 4, 2, 57, 55, 7], [2, 1, 4, 6, 5, 3], 8),
-                   ([24, 15, 4, 6, 2, 59], [6, 1, 4, 2, 5, 3], 4),
-                   ([2, 42, 2, 11, 11], [5, 2, 1, 4, 3], 8)]
+                   ([2, 3, 2, 11, 3, 39], [4, 3, 2, 6, 1, 5], 8),
+                   ([51, 6, 10, 10, 4], [1, 3, 2, 5, 4], 8),
+                   ([2, 39, 6, 2, 6, 13], [4, 5, 6, 3, 1, 2], 4)]
 
 
 @pytest.mark.parametrize("dims,perm1,elem", COMPOSITE_CASES)
