@@ -506,8 +506,10 @@ constexpr double kCostCycle = 1.15;  // 4.6-5.8 TB/s for cells of 128 B to 384 K
 constexpr double kCostBlock = 1.1;   // fused block permutation
 // Cells whose size is not a multiple of 16 bytes move in narrower words
 // (0.8-0.9 TB/s measured for 2- and 4-byte words).
+thread_local bool g_cycle_word_cost = true;  // off: every cell width costs the same
 inline double cycle_cost(int64_t U) {
   const int64_t w = pow2_factor(U, 16);
+  if (!g_cycle_word_cost) return kCostCycle;
   return w >= 16 ? kCostCycle : w == 8 ? 2.0 : 5.0;
 }
 
@@ -731,6 +733,7 @@ static int64_t* tuning_field(Tuning& t, const char* key) {
   if (!strcmp(key, "cycle_min_cell")) return &t.cycle_min_cell;
   if (!strcmp(key, "fuse_max_block")) return &t.fuse_max_block;
   if (!strcmp(key, "block_digits")) return &t.block_digits;
+  if (!strcmp(key, "cycle_word_cost")) return &t.cycle_word_cost;
   if (!strcmp(key, "threads")) return &t.threads;
   if (!strcmp(key, "threads_big")) return &t.threads_big;
   if (!strcmp(key, "xcd_remap")) return &t.xcd_remap;
@@ -772,6 +775,7 @@ void tuning_apply(const Tuning& t, DeviceProfile& prof) {
   prof.cycle_min_cell = t.cycle_min_cell;
   prof.fuse_max_block = std::min<int64_t>(t.fuse_max_block, prof.lds_max);
   prof.block_digits = t.block_digits;
+  prof.cycle_word_cost = t.cycle_word_cost;
   prof.reorder = (int)t.reorder;
 }
 
@@ -1135,6 +1139,7 @@ static void finalize_plan(Plan& plan, double alpha, int64_t reserve, const Devic
 
 int make_plan(const int64_t* dims_in, const int* perm1, int rank, size_t elem, int n_sub,
               double alpha, const DeviceProfile& prof, int reorder, Plan& plan) {
+  g_cycle_word_cost = prof.cycle_word_cost != 0;
   // Search: decomposer order (pre/post, or the one asked for) x how hard
   // staged passes are avoided; cycle passes that later compose with their
   // neighbours make a split lowering cheaper than a staged one, which only

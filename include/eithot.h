@@ -157,6 +157,8 @@ int eithot_plan_cycle_maps(const int64_t* dims, const int* perm, int order, size
  *   "block_digits"   fused blocks compute source cells from digits (1) or read
  *                    a gather table (0)
  *   "cycle_variant"  cycle-following kernel shape (0..7; 0 default)
+ *   "cycle_word_cost" cycle passes over cells that are not whole 16-byte
+ *                    words cost more in the planner (1) or the same (0)
  *   "threads", "threads_big"  workgroup sizes (256, 512; multiples of 64, at most 512)
  *   "xcd_remap"      keep consecutive footprints on one XCD (1)
  * Returns EITHOT_ERR_INVALID for an unknown key or an out-of-range value. */
