@@ -368,6 +368,14 @@ int64_t any_slice(int64_t units, int64_t U, int64_t cap) {
   return 0;
 }
 
+// A Catanzaro column or row pass moving `us`-byte cells: word-granular
+// gathers of small cells run far below HBM speed (0.4-1 TB/s measured for
+// 2- and 4-byte units), staged passes cross HBM twice.
+double cat_pass_cost(int64_t us, bool staged) {
+  const double f = us >= 16 ? 1.0 : us >= 8 ? 2.0 : us >= 4 ? 4.0 : 7.0;
+  return staged ? 2.0 * f : f;
+}
+
 struct CatPlan {
   bool ok = false;
   int64_t m, n, c, a, b;
@@ -429,8 +437,8 @@ CatPlan plan_catanzaro(int64_t R, int64_t C, int64_t U, const DeviceProfile& pro
   p.row_staged = false;
   p.row_us = pick_slice(p.n, U, prof.lds_max, prof.min_segment);
   if (!p.row_us) fallback(p.n, p.row_us, p.row_staged);
-  const double col_cost = p.col_staged ? 2.0 * prof.staged_mult : 1.0,
-               row_cost = p.row_staged ? 2.0 * prof.staged_mult : 1.0;
+  const double col_cost = cat_pass_cost(p.col_us, p.col_staged) * (p.col_staged ? prof.staged_mult : 1.0),
+               row_cost = cat_pass_cost(p.row_us, p.row_staged) * (p.row_staged ? prof.staged_mult : 1.0);
   p.cost = row_cost + col_cost * (p.npass - 1) + narrow_cost;
   // a staged pass needs at least one footprint of scratch
   const int64_t col_fp = p.m * p.col_W * p.col_us, row_fp = p.n * p.row_us;
@@ -499,6 +507,13 @@ constexpr double kCostTileStaged = 2.05;
 constexpr double kCostTileStagedSliced = 2.4;
 constexpr double kCostSplit = 0.001;
 
+// Staged tiles of cells under 16 bytes transpose word by word (about 0.7
+// TB/s measured on small cells).
+inline double staged_tile_cost(int64_t U, int64_t us) {
+  const double c = us == U ? kCostTileStaged : kCostTileStagedSliced;
+  return us < 16 ? 3.0 * c : c;
+}
+
 // Cycle-following moves whole cells (of at least cycle_min_cell bytes), one
 // read and one write each; smaller cells waste HBM bursts.
 constexpr uint64_t kCycleMaxCells = 1ull << 26;
@@ -510,7 +525,7 @@ thread_local bool g_cycle_word_cost = true;  // off: every cell width costs the 
 inline double cycle_cost(int64_t U) {
   const int64_t w = pow2_factor(U, 16);
   if (!g_cycle_word_cost) return kCostCycle;
-  return w >= 16 ? kCostCycle : w == 8 ? 2.0 : 5.0;
+  return w >= 16 ? kCostCycle : w == 8 ? 2.0 : w == 4 ? 4.0 : 6.0;
 }
 
 struct Choice {
@@ -559,7 +574,7 @@ struct Lowerer {
     // whole matrices (or slices) through scratch: copy out, tiled transpose back
     if (ch.cost > kCostTileStaged * prof.staged_mult) {
       const int64_t us = pick_staged_tile_slice(R, C, U, scratch_budget, prof);
-      const double cost = (us == U ? kCostTileStaged : kCostTileStagedSliced) * prof.staged_mult;
+      const double cost = staged_tile_cost(U, us) * prof.staged_mult;
       if (us && cost < ch.cost) {
         ch.how = HOW_TILE_STAGED;
         ch.cost = cost;
@@ -974,11 +989,11 @@ double plan_cost(const Plan& plan) {
   for (const Op& o : plan.ops) {
     switch (o.kind) {
       case OP_TILE:
-        c += o.staged ? (o.us == o.U ? kCostTileStaged : kCostTileStagedSliced)
+        c += o.staged ? staged_tile_cost(o.U, o.us)
                       : (o.us == o.U ? kCostTile : kCostTileSliced);
         break;
       case OP_COL:
-      case OP_ROW: c += o.staged ? 2.0 : 1.0; break;
+      case OP_ROW: c += cat_pass_cost(o.us, o.staged != 0); break;
       case OP_CYCLE: c += cycle_cost(o.U); break;
       case OP_BLOCK: c += kCostBlock; break;
     }
