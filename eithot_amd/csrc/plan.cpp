@@ -497,21 +497,23 @@ enum How {
 
 // Relative time of one pass over the tensor, by pass class (contiguous LDS
 // tile = 1, 5.7 TB/s of reads + writes on MI355X), from measurements on
-// 4.5 GB tensors: sliced LDS tiles move short scattered segments (2.3x),
-// staged tiles cross HBM twice (2.05x, 2.4x when sliced), cycle following
+// 4.5-172 GB tensors: sliced LDS tiles move short scattered segments (2.3x),
+// staged tiles cross HBM twice (2.05-2.5x, 2.4x when sliced), cycle following
 // reads and writes each cell once at scattered positions (1.6x, 1.75x for
 // cells under 1 KiB); Catanzaro passes evaluate heavier index maps.
 constexpr double kCostTile = 1.0;
 constexpr double kCostTileSliced = 2.3;
-constexpr double kCostTileStaged = 2.05;
+constexpr double kCostTileStaged = 2.2;
 constexpr double kCostTileStagedSliced = 2.4;
 constexpr double kCostSplit = 0.001;
 
-// Staged tiles of cells under 16 bytes transpose word by word (about 0.7
-// TB/s measured on small cells).
-inline double staged_tile_cost(int64_t U, int64_t us) {
+// Staged tiles of cells under 16 bytes whose output columns do not pack
+// into 16-byte stores transpose word by word (about 0.7 TB/s measured).
+inline double staged_tile_cost(int64_t R, int64_t U, int64_t us) {
   const double c = us == U ? kCostTileStaged : kCostTileStagedSliced;
-  return us < 16 ? 3.0 * c : c;
+  const int64_t w = pow2_factor(us, 16);
+  const bool packs = us == w && R % (16 / w) == 0;
+  return us < 16 && !packs ? 3.0 * c : c;
 }
 
 // Cycle-following moves whole cells (of at least cycle_min_cell bytes), one
@@ -574,7 +576,7 @@ struct Lowerer {
     // whole matrices (or slices) through scratch: copy out, tiled transpose back
     if (ch.cost > kCostTileStaged * prof.staged_mult) {
       const int64_t us = pick_staged_tile_slice(R, C, U, scratch_budget, prof);
-      const double cost = staged_tile_cost(U, us) * prof.staged_mult;
+      const double cost = staged_tile_cost(R, U, us) * prof.staged_mult;
       if (us && cost < ch.cost) {
         ch.how = HOW_TILE_STAGED;
         ch.cost = cost;
@@ -989,7 +991,7 @@ double plan_cost(const Plan& plan) {
   for (const Op& o : plan.ops) {
     switch (o.kind) {
       case OP_TILE:
-        c += o.staged ? staged_tile_cost(o.U, o.us)
+        c += o.staged ? staged_tile_cost(o.R, o.U, o.us)
                       : (o.us == o.U ? kCostTile : kCostTileSliced);
         break;
       case OP_COL:
