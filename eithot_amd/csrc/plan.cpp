@@ -756,6 +756,7 @@ static int64_t* tuning_field(Tuning& t, const char* key) {
   if (!strcmp(key, "xcd_remap")) return &t.xcd_remap;
   if (!strcmp(key, "cycle_variant")) return &t.cycle_variant;
   if (!strcmp(key, "reorder")) return &t.reorder;
+  if (!strcmp(key, "regroup")) return &t.regroup;
   return nullptr;
 }
 
@@ -794,6 +795,7 @@ void tuning_apply(const Tuning& t, DeviceProfile& prof) {
   prof.block_digits = t.block_digits;
   prof.cycle_word_cost = t.cycle_word_cost;
   prof.reorder = (int)t.reorder;
+  prof.regroup = t.regroup;
 }
 
 void lower_transpose2d(int64_t B, int64_t R, int64_t C, int64_t U, uint64_t base, int stage,
@@ -1058,6 +1060,174 @@ void fuse_tiles(std::vector<Op>& ops, const DeviceProfile& prof) {
   ops.swap(out);
 }
 
+Stage make_stage(int64_t P, int64_t X, int64_t Y, int64_t S) {
+  Stage s;
+  s.P = P;
+  s.X = X;
+  s.Y = Y;
+  s.S = S;
+  int n = 0;
+  if (P > 1) s.dims[n++] = P;
+  s.dims[n++] = X;
+  s.dims[n++] = Y;
+  if (S > 1) s.dims[n++] = S;
+  s.rank = n;
+  const int xi = P > 1 ? 2 : 1;
+  n = 0;
+  if (P > 1) s.perm[n++] = 1;
+  s.perm[n++] = xi + 1;
+  s.perm[n++] = xi;
+  if (S > 1) s.perm[n++] = xi + 2;
+  return s;
+}
+
+// A transpose whose fastest dim moves, as three steps (the decomposer's
+// stage order is one plan among many; this family often needs fewer passes):
+//   1. cells of the input's leading dims X move so that the output's leading
+//      dims Y follow X, the other dims Z after them  (X, Y, Z);
+//   2. one batched 2-D transpose of the X-by-Y matrices  (Y, X, Z);
+//   3. cells of Y move into the output order of X and Z.
+// Steps 1 and 3 are whole-cell permutations: decomposed like any
+// permutation, lowered, and composed into single cycle passes by
+// fuse_cycles; step 2 is a tile pass when an X-by-Y matrix fits LDS.
+struct Regroup {
+  std::vector<int64_t> q;  // canonical dims, fastest first
+  std::vector<int> p;      // canonical 0-based perm: output k = input p[k]
+  int64_t E = 0;           // bytes of the canonical element
+
+  // Drops unit dims, merges dims adjacent in both orders, folds a leading
+  // dim that stays in place into the element.  False when nothing is left
+  // to transpose.
+  bool canonical(const int64_t* dims, const int* perm1, int rank, size_t elem) {
+    std::vector<int> keep(rank, -1);
+    std::vector<int64_t> d;
+    for (int i = 0; i < rank; ++i)
+      if (dims[i] > 1) {
+        keep[i] = (int)d.size();
+        d.push_back(dims[i]);
+      }
+    std::vector<int> pp;
+    for (int k = 0; k < rank; ++k)
+      if (keep[perm1[k] - 1] >= 0) pp.push_back(keep[perm1[k] - 1]);
+    // merge runs: input dim a followed by a+1 in the output too
+    const int n0 = (int)d.size();
+    std::vector<int> head(n0, 1);  // 1 when input dim a starts a run
+    for (int k = 0; k + 1 < (int)pp.size(); ++k)
+      if (pp[k + 1] == pp[k] + 1) head[pp[k + 1]] = 0;
+    std::vector<int> run(n0);
+    q.clear();
+    for (int a = 0; a < n0; ++a) {
+      if (head[a]) q.push_back(1);
+      run[a] = (int)q.size() - 1;
+      q.back() *= d[a];
+    }
+    p.clear();
+    for (int k = 0; k < (int)pp.size(); ++k)
+      if (head[pp[k]]) p.push_back(run[pp[k]]);
+    E = (int64_t)elem;
+    if (!p.empty() && p[0] == 0) {
+      E *= q[0];
+      q.erase(q.begin());
+      p.erase(p.begin());
+      for (int& v : p) --v;
+    }
+    return q.size() >= 2;
+  }
+};
+
+// Lowered ops of one whole-cell permutation (cell dims fastest first,
+// 1-based perm), the cheaper decomposer order.
+double lower_cellperm(Lowerer& L, const std::vector<int64_t>& cd, const std::vector<int>& perm1,
+                      int64_t cell, int stage0, const DeviceProfile& prof, std::vector<Op>& ops,
+                      std::vector<Stage>& stages) {
+  double best = 1e300;
+  for (int order : {(int)REORDER_PRE, (int)REORDER_POST}) {
+    std::vector<Stage> st;
+    if (!decompose(cd.data(), perm1.data(), (int)cd.size(), order, st)) return 1e300;
+    std::vector<Op> o;
+    for (size_t s = 0; s < st.size(); ++s)
+      L.emit(st[s].S, st[s].Y, st[s].X, st[s].P * cell, 0, stage0 + (int)s, 0, o);
+    bool ok = true;
+    for (const Op& op : o) ok = ok && op.kind != 0;
+    if (!ok) continue;
+    fuse_cycles(o, prof);
+    Plan tmp;
+    tmp.ops = o;
+    const double c = plan_cost(tmp);
+    if (c < best - 1e-9) {
+      best = c;
+      ops = std::move(o);
+      stages = std::move(st);
+    }
+  }
+  return best;
+}
+
+// Best plan of the regroup family for (X = the first i canonical input
+// dims, Y = the first j output dims); false when none lowers.
+bool regroup_plan(const Regroup& g, int i, int j, bool z_out_order, size_t elem, Lowerer& L,
+                  const DeviceProfile& prof, Plan& plan) {
+  const int n = (int)g.q.size();
+  std::vector<int> inX(n, 0), inY(n, 0);
+  int64_t px = 1, py = 1, pz = 1;
+  for (int a = 0; a < i; ++a) {
+    inX[a] = 1;
+    px *= g.q[a];
+  }
+  for (int k = 0; k < j; ++k) {
+    if (inX[g.p[k]]) return false;
+    inY[g.p[k]] = 1;
+    py *= g.q[g.p[k]];
+  }
+  std::vector<int> Z;
+  if (z_out_order) {
+    for (int k = j; k < n; ++k)
+      if (!inX[g.p[k]]) Z.push_back(g.p[k]);
+  } else {
+    for (int a = i; a < n; ++a)
+      if (!inY[a]) Z.push_back(a);
+  }
+  for (int a : Z) pz *= g.q[a];
+  // step 1: cell dims i..n-1 (input order) -> (Y in output order, Z)
+  std::vector<int64_t> c1;
+  for (int a = i; a < n; ++a) c1.push_back(g.q[a]);
+  std::vector<int> t1;
+  for (int k = 0; k < j; ++k) t1.push_back(g.p[k] - i + 1);
+  for (int a : Z) t1.push_back(a - i + 1);
+  // step 3: cell dims (X in input order, Z) -> output order of the rest
+  std::vector<int> l3;
+  for (int a = 0; a < i; ++a) l3.push_back(a);
+  l3.insert(l3.end(), Z.begin(), Z.end());
+  std::vector<int64_t> c3;
+  for (int a : l3) c3.push_back(g.q[a]);
+  std::vector<int> t3;
+  for (int k = j; k < n; ++k)
+    t3.push_back((int)(std::find(l3.begin(), l3.end(), g.p[k]) - l3.begin()) + 1);
+
+  Plan out;
+  std::vector<Op> o1, o3;
+  std::vector<Stage> s1, s3;
+  if (lower_cellperm(L, c1, t1, px * g.E, 0, prof, o1, s1) >= 1e300) return false;
+  std::vector<Op> o2;
+  L.emit(pz, py, px, g.E, 0, (int)s1.size(), 0, o2);
+  for (const Op& op : o2)
+    if (op.kind == 0) return false;
+  if (lower_cellperm(L, c3, t3, py * g.E, (int)s1.size() + 1, prof, o3, s3) >= 1e300)
+    return false;
+  const int64_t ue = g.E / (int64_t)elem;  // canonical element in tensor elements
+  for (const Stage& s : s1) out.stages.push_back(make_stage(s.P * px * ue, s.X, s.Y, s.S));
+  out.stages.push_back(make_stage(ue, px, py, pz));
+  for (const Stage& s : s3) out.stages.push_back(make_stage(s.P * py * ue, s.X, s.Y, s.S));
+  out.ops = std::move(o1);
+  out.ops.insert(out.ops.end(), o2.begin(), o2.end());
+  out.ops.insert(out.ops.end(), o3.begin(), o3.end());
+  fuse_cycles(out.ops, prof);
+  fuse_tiles(out.ops, prof);
+  plan.stages = std::move(out.stages);
+  plan.ops = std::move(out.ops);
+  return true;
+}
+
 }  // namespace
 
 // `reserve`: bytes of the alpha budget kept back for the pass tables.
@@ -1195,6 +1365,18 @@ int make_plan(const int64_t* dims_in, const int* perm1, int rank, size_t elem, i
   double best_cost = 1e300;
   int best_order = orders[0];
   double best_mult = 1.0;
+  // the regroup family, unless the caller asked for sub-tensor partitioning
+  // and it applies
+  bool partition = false;
+  if (vol_ok && n_sub > 1 && alpha > 0) {
+    int pd = 0;
+    for (int i = 1; i < rank; ++i)
+      if (dims_in[i] > dims_in[pd]) pd = i;
+    partition = dims_in[pd] > (int64_t)(32.0 * (1.0 / alpha));
+  }
+  Regroup rg;
+  const bool regroup = vol_ok && !partition && reorder == 0 && prof.regroup &&
+                       rg.canonical(dims_in, perm1, rank, elem) && rg.q.size() >= 3;
   for (double mult : {1.0, 3.0}) {
     DeviceProfile p2 = prof;
     p2.staged_mult = mult;
@@ -1216,6 +1398,23 @@ int make_plan(const int64_t* dims_in, const int* perm1, int rank, size_t elem, i
       }
       rc = EITHOT_OK;
     }
+    if (!regroup || (rc && rc != EITHOT_ERR_ALPHA)) continue;
+    const int n = (int)rg.q.size();
+    for (int i = 1; i < n && i <= 6; ++i)
+      for (int j = 1; i + j <= n && j <= 6; ++j)
+        for (bool zo : {false, true}) {
+          if (zo && i + j == n) continue;  // no Z: both orders agree
+          Plan q;
+          if (!regroup_plan(rg, i, j, zo, elem, shared, p2, q)) continue;
+          q.tensor_bytes = vol * elem;
+          const double c = plan_cost(q);
+          if (rc || c < best_cost - 1e-9) {
+            best_cost = c;
+            best_mult = mult;
+            plan = std::move(q);
+            rc = EITHOT_OK;
+          }
+        }
   }
   if (rc) return rc;
   DeviceProfile pb = prof;

@@ -147,12 +147,15 @@ int eithot_plan_cycle_maps(const int64_t* dims, const int* perm, int order, size
  * No counterpart in the reference, whose equivalents are compile-time
  * constants (REORDER_TYPE, Nd_transpose.cpp:28; tier thresholds in col_op.cu
  * and row_op.cu).  Process-wide knobs, applied to plans made after the call:
- *   "reorder"        decomposer order: 0 mix (default), 1 pre, 2 post
+ *   "reorder"        decomposer order: 0 plans pre and post and keeps the
+ *                    cheaper (default), 1 pre, 2 post, 3 mix (the reference's rule)
+ *   "regroup"        with reorder 0, also plans a cell pass, one tile pass and
+ *                    a cell pass around the moving fastest dims (1; 0 = off)
  *   "lds_target"     LDS bytes a workgroup packs footprints up to (65536)
  *   "tile_fp_target" tile footprints above this are sliced (65536; 0 = off)
  *   "slice_min"      smallest slice such slicing may choose (128)
  *   "stage_chunk"    > 0: bytes of scratch one staged group may fill (0)
- *   "cycle_min_cell" smallest cell (bytes) moved by cycle following (512; 0 = off)
+ *   "cycle_min_cell" smallest cell (bytes) moved by cycle following (128; 0 = off)
  *   "fuse_max_block" largest block of a fused tile permutation (65536; 0 = off)
  *   "block_digits"   fused blocks compute source cells from digits (1) or read
  *                    a gather table (0)

@@ -113,6 +113,7 @@ def test_fused_block_passes(torch_cuda):
              ([63, 2, 1001, 3], [2, 1, 3, 4], 1), ([4, 2, 30011], [2, 1, 3], 1),
              ([5, 3, 7, 2001], [2, 1, 3, 4], 2), ([3, 9, 7, 1001], [1, 3, 2, 4], 1)]
     default = E.get_tuning("fuse_max_block")
+    regroup = E.set_tuning("regroup", 0)  # keep the plans that carry block passes
     try:
         for fuse, digits in ((default, 1), (default, 0), (0, 1)):
             E.set_tuning("fuse_max_block", fuse)
@@ -126,6 +127,35 @@ def test_fused_block_passes(torch_cuda):
     finally:
         E.set_tuning("fuse_max_block", default)
         E.set_tuning("block_digits", 1)
+        E.set_tuning("regroup", regroup)
+
+
+REGROUP_CASES = [([15, 24, 20, 13, 25], [5, 3, 1, 2, 4], 4), ([30, 9, 2, 66, 63], [5, 4, 1, 3, 2], 4),
+                 ([13, 13, 23, 12, 2, 22], [4, 2, 6, 5, 1, 3], 4),
+                 ([10, 37, 6, 30, 27], [5, 4, 3, 2, 1], 1),
+                 ([12, 15, 11, 11, 13, 10], [2, 5, 3, 6, 4, 1], 8),
+                 ([35, 27, 5, 3, 3, 8], [6, 4, 2, 5, 1, 3], 4),
+                 ([2, 4, 103, 21, 117], [5, 3, 2, 4, 1], 8),
+                 ([6, 2, 30, 25, 17, 18], [3, 6, 5, 1, 4, 2], 2),
+                 ([32, 48, 64, 5, 6, 8], [6, 4, 2, 5, 1, 3], 4)]
+
+
+def test_regroup_plans(torch_cuda):
+    # cell pass / tile pass / cell pass plans: each case plans differently
+    # with the regroup family off
+    import eithot_amd as E
+    key = lambda o: (o["kind"], o["B"], o["R"], o["C"], o["U"], o["us"])
+    for dims, perm, elem in REGROUP_CASES:
+        ops, _ = E.plan_describe(dims, perm, elem, 1, 0.05)
+        old = E.set_tuning("regroup", 0)
+        try:
+            ops0, _ = E.plan_describe(dims, perm, elem, 1, 0.05)
+        finally:
+            E.set_tuning("regroup", old)
+        assert [key(o) for o in ops] != [key(o) for o in ops0], (dims, perm)
+        got, want, peak = run_case(torch_cuda, dims, perm, elem, seed=12, alpha=0.05)
+        assert_bitexact(got, want, f"regroup {dims} {perm} elem {elem}")
+        assert peak <= max(0.05 * got.nbytes, 1 << 20)
 
 
 def test_random_high_order(torch_cuda):
