@@ -38,7 +38,7 @@ def inverse(dims, perm1):
 
 # HIP kernels that carry each pass class (staged passes launch two per group)
 HIP_KERNEL = {"cycle": "k_cycle", "tile": "k_tile", "tile_staged": "copy + k_tile_oop",
-              "block": "k_block"}
+              "block": "k_block", "segtile": "k_segtile"}
 for _m in ("_rot", "_rotinv", "_colsh", "_colshinv"):
     HIP_KERNEL["col" + _m] = "k_col"
     HIP_KERNEL["col" + _m + "_staged"] = "k_stage_out + k_stage_in"
@@ -48,7 +48,7 @@ for _m in ("_rowg", "_rowd"):
 
 
 def kernel_name(op):
-    kinds = {1: "tile", 2: "col", 3: "row", 4: "cycle", 5: "block"}
+    kinds = {1: "tile", 2: "col", 3: "row", 4: "cycle", 5: "block", 6: "segtile"}
     maps = {0: "", 1: "_rot", 2: "_rotinv", 3: "_colsh", 4: "_colshinv", 5: "_rowg", 6: "_rowd"}
     return f"{kinds[op['kind']]}{maps[op['map']]}{'_staged' if op['staged'] else ''}"
 

@@ -131,6 +131,16 @@ def plan_describe(dims, perm1, elem_bytes, n_sub=1, alpha=0.05, lds_max=0):
                 i64(dims), i32(perm1), len(dims), int(elem_bytes), int(n_sub), float(alpha),
                 int(lds_max), i, mp, 4))
             d["maps"] = [tuple(mp[4 * q:4 * q + 4]) for q in range(k)]
+        if d["kind"] == 6:  # segmented tile: attach its segment size and digits
+            sb = ctypes.c_int64()
+            inner = (ctypes.c_int64 * 12)()
+            outer = (ctypes.c_int64 * 12)()
+            k = check(lib().eithot_plan_segments(
+                i64(dims), i32(perm1), len(dims), int(elem_bytes), int(n_sub), float(alpha),
+                int(lds_max), i, ctypes.byref(sb), inner, outer))
+            d["segb"] = sb.value
+            d["inner"] = [(inner[2 * q], inner[2 * q + 1]) for q in range(k % 16)]
+            d["outer"] = [(outer[2 * q], outer[2 * q + 1]) for q in range(k // 16)]
         out.append(d)
     return out, scratch.value
 

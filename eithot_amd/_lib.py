@@ -27,7 +27,8 @@ EXPORTS = [
     "eithot_transpose", "eithot_transpose_async", "eithot_inplace_transpose",
     "eithot_last_peak_extra_bytes", "eithot_last_error", "eithot_release", "eithot_estimate",
     "eithot_estimate_reference", "eithot_decompose", "eithot_plan_describe",
-    "eithot_plan_block_table", "eithot_plan_cycle_maps", "eithot_set_tuning",
+    "eithot_plan_block_table", "eithot_plan_cycle_maps", "eithot_plan_segments",
+    "eithot_set_tuning",
     "eithot_get_tuning",
     "eithot_set_pass_timing", "eithot_last_pass_times", "eithot_fill_pattern",
     "eithot_verify_pattern", "eithot_batch_create", "eithot_batch_add", "eithot_batch_build",
@@ -95,6 +96,8 @@ def lib():
                                           ctypes.POINTER(ctypes.c_uint32), ctypes.c_int64]
     L.eithot_plan_cycle_maps.argtypes = [i64p, ip, ctypes.c_int, sz, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_int64, ctypes.c_int, i64p, ctypes.c_int]
+    L.eithot_plan_segments.argtypes = [i64p, ip, ctypes.c_int, sz, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_int64, ctypes.c_int, i64p, i64p, i64p]
     L.eithot_set_tuning.argtypes = [ctypes.c_char_p, ctypes.c_int64]
     L.eithot_get_tuning.argtypes = [ctypes.c_char_p, i64p]
     L.eithot_set_pass_timing.argtypes = [ctypes.c_int]

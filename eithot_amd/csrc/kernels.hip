@@ -1002,6 +1002,12 @@ __global__ void k_verify(const T* p, VerifyArgs v, uint64_t seed_mixed, int mode
 int64_t op_lds_bytes(const Op& op, const LaunchCfg& cfg) {
   if (op.staged || op.kind == 0 || op.kind == OP_CYCLE) return 0;
   if (op.kind == OP_BLOCK) return op.blk ? (op.blk->lds_bytes() + 15) / 16 * 16 : op.R * op.U;
+  if (op.kind == OP_SEGTILE) {
+    int64_t uw = 1;
+    while (uw < 16 && op.U % (uw * 2) == 0) uw *= 2;
+    return (op.R * tile_row_stride(op.C * op.U, uw) + 15) / 16 * 16 +
+           (op.segb > 0 ? op.R * op.C * op.U / op.segb * 8 : 0);
+  }
   Args<uint64_t> a;
   Prepared p;
   prepare_lds<uint64_t>(op, nullptr, cfg, a, p);
@@ -1617,6 +1623,286 @@ hipError_t launch_block(const Op& op, char* data, const LaunchCfg& cfg, hipStrea
 }
 
 // ---------------------------------------------------------------------------
+// segmented tile pass (OP_SEGTILE): K matrices per workgroup.  A matrix of
+// R x C units of E bytes (C fastest) lies in nseg segments of segb bytes at
+// addresses given by mixed-radix digits.  The workgroup writes the K*nseg
+// segment addresses to LDS, loads its matrices row by row into LDS (rows
+// padded to rs bytes), and stores the transposed matrices, in row-major
+// order, over the same segments.  Words of w bytes never straddle a
+// segment, an LDS row or (when V > 1) an output row: w divides segb, C*E
+// and R*E.  V > 1: one word packs V units (E = w / V) of an output row;
+// V = 1: a unit is E / w words.
+// ---------------------------------------------------------------------------
+struct SegArgs {
+  char* data;
+  uint32_t B, R, C, E;   // matrices; rows, columns and unit bytes of each
+  uint32_t nseg;         // segments per matrix
+  uint32_t K;            // matrices per workgroup
+  uint32_t rs, fpl;      // LDS bytes per padded row, per matrix
+  uint32_t tbl;          // LDS byte offset of the segment addresses
+  uint32_t wpm;          // words per matrix
+  uint32_t nin, nout;    // digits of the segment index and of the matrix index
+  uint64_t sin[6], sout[6];
+  FD32 fin[6], fout[6];  // their radices
+  FD32 fnseg, fwpm, fwps, fwpr, fwpo, fwpe;  // nseg; words per matrix, segment,
+                                             // LDS row, output row, unit
+};
+
+template <int BYTES>
+struct UIntOf;
+template <>
+struct UIntOf<1> {
+  using T = uint8_t;
+};
+template <>
+struct UIntOf<2> {
+  using T = uint16_t;
+};
+template <>
+struct UIntOf<4> {
+  using T = uint32_t;
+};
+template <>
+struct UIntOf<8> {
+  using T = uint64_t;
+};
+
+// LDS word store / load at an offset aligned to min(4, word) bytes
+template <typename Word>
+__device__ __forceinline__ void lds_putw(unsigned char* p, const Word& v) {
+  if constexpr (sizeof(Word) == 16) {
+    lds_put16(p, v);
+  } else if constexpr (sizeof(Word) == 8) {
+    if (((uintptr_t)p & 7) == 0) {
+      *reinterpret_cast<Word*>(p) = v;
+    } else {
+      reinterpret_cast<uint32_t*>(p)[0] = (uint32_t)v;
+      reinterpret_cast<uint32_t*>(p)[1] = (uint32_t)(v >> 32);
+    }
+  } else {
+    *reinterpret_cast<Word*>(p) = v;
+  }
+}
+template <typename Word>
+__device__ __forceinline__ Word lds_getw(const unsigned char* p) {
+  if constexpr (sizeof(Word) >= 8) {
+    if (((uintptr_t)p & (sizeof(Word) - 1)) == 0) return *reinterpret_cast<const Word*>(p);
+    union {
+      uint32_t u[sizeof(Word) / 4];
+      Word w;
+    } x;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(Word) / 4); ++i) x.u[i] = reinterpret_cast<const uint32_t*>(p)[i];
+    return x.w;
+  } else {
+    return *reinterpret_cast<const Word*>(p);
+  }
+}
+
+__device__ __forceinline__ uint64_t seg_addr(const SegArgs& a, uint32_t s, uint32_t b) {
+  uint64_t addr = 0;
+#pragma unroll
+  for (int d = 0; d < 6; ++d) {
+    if (d >= (int)a.nin) break;
+    const uint32_t q = a.fin[d].div(s);
+    addr += (uint64_t)(s - q * a.fin[d].d) * a.sin[d];
+    s = q;
+  }
+#pragma unroll
+  for (int d = 0; d < 6; ++d) {
+    if (d >= (int)a.nout) break;
+    const uint32_t q = a.fout[d].div(b);
+    addr += (uint64_t)(b - q * a.fout[d].d) * a.sout[d];
+    b = q;
+  }
+  return addr;
+}
+
+// f0: first matrix of the workgroup's K
+template <typename Word, int V>
+__device__ __forceinline__ void seg_body(const SegArgs& a, const uint32_t f0, unsigned char* lds) {
+  constexpr uint32_t w = sizeof(Word);
+  const uint32_t nf = (a.B - f0) < a.K ? (a.B - f0) : a.K;
+  const uint32_t tid = threadIdx.x, nt = blockDim.x;
+  uint64_t* tbl = reinterpret_cast<uint64_t*>(lds + a.tbl);
+  for (uint32_t e = tid; e < nf * a.nseg; e += nt) {
+    const uint32_t k = a.fnseg.div(e);
+    tbl[e] = seg_addr(a, e - k * a.nseg, f0 + k);
+  }
+  __syncthreads();
+  const uint32_t n = nf * a.wpm;
+  for (uint32_t t0 = tid; t0 < n; t0 += nt * UNR) {
+    Word v[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const uint32_t t = t0 + u * nt;
+      if (t < n) {
+        const uint32_t k = a.fwpm.div(t), q = t - k * a.wpm;  // word q of matrix k
+        const uint32_t s = a.fwps.div(q);
+        v[u] = *reinterpret_cast<const Word*>(a.data + tbl[k * a.nseg + s] +
+                                              (uint64_t)(q - s * a.fwps.d) * w);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const uint32_t t = t0 + u * nt;
+      if (t < n) {
+        const uint32_t k = a.fwpm.div(t), q = t - k * a.wpm;
+        const uint32_t r = a.fwpr.div(q);
+        lds_putw<Word>(lds + k * a.fpl + r * a.rs + (q - r * a.fwpr.d) * w, v[u]);
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t t = tid; t < n; t += nt) {
+    const uint32_t k = a.fwpm.div(t), q = t - k * a.wpm;  // output word q of matrix k
+    const uint32_t j = a.fwpo.div(q), x = q - j * a.fwpo.d;  // output row j (column j in LDS)
+    const unsigned char* l = lds + k * a.fpl + j * a.E;
+    Word val;
+    if constexpr (V > 1) {
+      using Elem = typename UIntOf<sizeof(Word) / V>::T;
+      union {
+        Elem e[V];
+        Word w;
+      } pk;
+#pragma unroll
+      for (int v = 0; v < V; ++v) pk.e[v] = *reinterpret_cast<const Elem*>(l + (x * V + v) * a.rs);
+      val = pk.w;
+    } else {
+      const uint32_t i = a.fwpe.div(x);
+      val = lds_getw<Word>(l + i * a.rs + (x - i * a.fwpe.d) * w);
+    }
+    const uint32_t s = a.fwps.div(q);
+    *reinterpret_cast<Word*>(a.data + tbl[k * a.nseg + s] + (uint64_t)(q - s * a.fwps.d) * w) = val;
+  }
+}
+
+template <typename Word, int V>
+__global__ __launch_bounds__(512) void k_segtile(SegArgs a, int remap) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  seg_body<Word, V>(a, xcd_logical(blockIdx.x, gridDim.x, remap) * a.K, lds);
+}
+
+// Word size of a segmented tile pass: the widest (at most 16 bytes) that
+// divides the segments, both row lengths and the data pointer, and that
+// divides the unit or is a multiple of it.
+int64_t segtile_word_bytes(const Op& op, uintptr_t data) {
+  int64_t w = 16;
+  while (w > 1 && ((op.segb % w) || ((op.C * op.U) % w) || ((op.R * op.U) % w) || (data % w)))
+    w /= 2;
+  while (w > 1 && (op.U % w) && (w % op.U)) w /= 2;
+  return w;
+}
+
+// Arguments of a segmented tile pass; *w: word bytes, *V: units per word,
+// *lds: LDS bytes per workgroup, *groups: workgroups of K matrices.
+hipError_t prepare_seg_args(const Op& op, char* data, const LaunchCfg& cfg, SegArgs& a,
+                            int64_t* w_out, int64_t* V_out, int64_t* lds_out,
+                            int64_t* groups_out) {
+  const int64_t bytes = op.R * op.C * op.U;
+  if (op.kind != OP_SEGTILE || op.segb <= 0 || op.B <= 0 || op.B >= (1ll << 32) ||
+      bytes % op.segb || op.sin_n > 6 || op.sout_n > 6 || op.base != 0)
+    return hipErrorInvalidValue;
+  // the digits must number exactly nseg segments and B matrices, and the
+  // farthest segment must end inside the B*R*C*U bytes of the tensor
+  {
+    unsigned __int128 pin = 1, pout = 1, last = (unsigned __int128)op.segb;
+    for (int d = 0; d < op.sin_n; ++d) {
+      if (op.sin_radix[d] < 1 || op.sin_stride[d] < 0) return hipErrorInvalidValue;
+      pin *= (uint64_t)op.sin_radix[d];
+      last += (unsigned __int128)(op.sin_radix[d] - 1) * (uint64_t)op.sin_stride[d];
+    }
+    for (int d = 0; d < op.sout_n; ++d) {
+      if (op.sout_radix[d] < 1 || op.sout_stride[d] < 0) return hipErrorInvalidValue;
+      pout *= (uint64_t)op.sout_radix[d];
+      last += (unsigned __int128)(op.sout_radix[d] - 1) * (uint64_t)op.sout_stride[d];
+    }
+    if (pin != (unsigned __int128)(bytes / op.segb) || pout != (unsigned __int128)op.B ||
+        last > (unsigned __int128)op.B * (uint64_t)bytes)
+      return hipErrorInvalidValue;
+  }
+  const int64_t w = segtile_word_bytes(op, (uintptr_t)data);
+  const int64_t V = w > op.U ? w / op.U : 1;
+  int64_t uw = 1;  // largest power of two (at most 16) dividing the unit
+  while (uw < 16 && op.U % (uw * 2) == 0) uw *= 2;
+  const int64_t rs = tile_row_stride(op.C * op.U, uw);
+  const int64_t fpl = (op.R * rs + 15) / 16 * 16;
+  const int64_t nseg = bytes / op.segb;
+  if (fpl + nseg * 8 > 160 * 1024) return hipErrorInvalidValue;
+  int64_t K = pick_K(op.B, fpl + nseg * 8, cfg.lds_target);
+  while (K > 1 && K * (fpl + nseg * 8) > 160 * 1024) --K;
+  std::memset(&a, 0, sizeof(a));
+  a.data = data;
+  a.B = (uint32_t)op.B;
+  a.R = (uint32_t)op.R;
+  a.C = (uint32_t)op.C;
+  a.E = (uint32_t)op.U;
+  a.nseg = (uint32_t)nseg;
+  a.K = (uint32_t)K;
+  a.rs = (uint32_t)rs;
+  a.fpl = (uint32_t)fpl;
+  a.tbl = (uint32_t)(K * fpl);
+  a.wpm = (uint32_t)(bytes / w);
+  a.nin = (uint32_t)op.sin_n;
+  a.nout = (uint32_t)op.sout_n;
+  for (int d = 0; d < 6; ++d) {
+    a.fin[d].init(d < op.sin_n ? op.sin_radix[d] : 1);
+    a.sin[d] = d < op.sin_n ? (uint64_t)op.sin_stride[d] : 0;
+    a.fout[d].init(d < op.sout_n ? op.sout_radix[d] : 1);
+    a.sout[d] = d < op.sout_n ? (uint64_t)op.sout_stride[d] : 0;
+  }
+  a.fnseg.init(nseg);
+  a.fwpm.init(bytes / w);
+  a.fwps.init(op.segb / w);
+  a.fwpr.init(op.C * op.U / w);
+  a.fwpo.init(op.R * op.U / w);
+  a.fwpe.init(V > 1 ? 1 : op.U / w);
+  const int64_t groups = (op.B + K - 1) / K;
+  if (groups >= (1ll << 31)) return hipErrorInvalidValue;
+  *w_out = w;
+  *V_out = V;
+  *lds_out = K * (fpl + nseg * 8);
+  *groups_out = groups;
+  return hipSuccess;
+}
+
+template <typename Word, int V>
+hipError_t launch_seg_wv(const SegArgs& a, int64_t lds, int64_t groups, const LaunchCfg& cfg,
+                         hipStream_t stream) {
+  const int threads = lds > 96 * 1024 ? cfg.threads_big : cfg.threads;
+  hipLaunchKernelGGL((k_segtile<Word, V>), dim3((uint32_t)groups), dim3(threads), (size_t)lds,
+                     stream, a, cfg.xcd_remap);
+  return hipGetLastError();
+}
+
+template <typename Word>
+hipError_t launch_seg_w(const SegArgs& a, int64_t V, int64_t lds, int64_t groups,
+                        const LaunchCfg& cfg, hipStream_t stream) {
+  constexpr int64_t w = sizeof(Word);
+  switch (V) {
+    case 1: return launch_seg_wv<Word, 1>(a, lds, groups, cfg, stream);
+    case 2: return launch_seg_wv<Word, (w >= 2 ? 2 : 1)>(a, lds, groups, cfg, stream);
+    case 4: return launch_seg_wv<Word, (w >= 4 ? 4 : 1)>(a, lds, groups, cfg, stream);
+    case 8: return launch_seg_wv<Word, (w >= 8 ? 8 : 1)>(a, lds, groups, cfg, stream);
+    default: return launch_seg_wv<Word, (w >= 16 ? 16 : 1)>(a, lds, groups, cfg, stream);
+  }
+}
+
+hipError_t launch_segtile(const Op& op, char* data, const LaunchCfg& cfg, hipStream_t stream) {
+  SegArgs a;
+  int64_t w = 1, V = 1, lds = 0, groups = 0;
+  hipError_t e = prepare_seg_args(op, data, cfg, a, &w, &V, &lds, &groups);
+  if (e != hipSuccess) return e;
+  switch (w) {
+    case 16: return launch_seg_w<W16>(a, V, lds, groups, cfg, stream);
+    case 8: return launch_seg_w<uint64_t>(a, V, lds, groups, cfg, stream);
+    case 4: return launch_seg_w<uint32_t>(a, V, lds, groups, cfg, stream);
+    case 2: return launch_seg_w<uint16_t>(a, V, lds, groups, cfg, stream);
+    default: return launch_seg_w<uint8_t>(a, V, lds, groups, cfg, stream);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // persistent batched launch: items of many tensors' passes in one kernel
 // ---------------------------------------------------------------------------
 // An op is the arguments of one tensor's pass; its variant selects the body:
@@ -1733,6 +2019,7 @@ hipError_t launch_op(const Op& op, char* data, char* scratch, int64_t scratch_by
   if (op.kind == OP_CYCLE)
     return launch_cycle(op, data, scratch, scratch_bytes, cfg.cycle_variant, stream);
   if (op.kind == OP_BLOCK) return launch_block(op, data, cfg, stream);
+  if (op.kind == OP_SEGTILE) return launch_segtile(op, data, cfg, stream);
   if (op.kind == 0 || op.us <= 0 || op.U % op.us) return hipErrorInvalidValue;
   if (needs64(op)) return run_op<uint64_t>(op, data, scratch, scratch_bytes, cfg, stream);
   return run_op<uint32_t>(op, data, scratch, scratch_bytes, cfg, stream);
@@ -2018,6 +2305,12 @@ void cfg_word(int64_t bytes, hipError_t& e) {
   set((const void*)k_tile_oop<Word, uint64_t, 1>);
   set((const void*)k_block<Word, 1>);
   if (V > 1) set((const void*)k_block<Word, V>);
+  constexpr int w = sizeof(Word);
+  set((const void*)k_segtile<Word, 1>);
+  if (w >= 2) set((const void*)k_segtile<Word, (w >= 2 ? 2 : 1)>);
+  if (w >= 4) set((const void*)k_segtile<Word, (w >= 4 ? 4 : 1)>);
+  if (w >= 8) set((const void*)k_segtile<Word, (w >= 8 ? 8 : 1)>);
+  if (w >= 16) set((const void*)k_segtile<Word, (w >= 16 ? 16 : 1)>);
 }
 }  // namespace
 

@@ -196,6 +196,7 @@ int64_t Op::footprint_units() const {
     case OP_ROW: return C;
     case OP_CYCLE: return 1;  // scratch: one saved cell per boundary slot
     case OP_BLOCK: return R;
+    case OP_SEGTILE: return R * C;
   }
   return 0;
 }
@@ -208,6 +209,7 @@ int64_t Op::num_footprints() const {
     case OP_ROW: return B * R * slices;
     case OP_CYCLE: return cyc ? B * (int64_t)cyc->slots.size() : 0;
     case OP_BLOCK: return B;
+    case OP_SEGTILE: return B;
   }
   return 0;
 }
@@ -528,6 +530,20 @@ inline double cycle_cost(int64_t U) {
   const int64_t w = pow2_factor(U, 16);
   if (!g_cycle_word_cost) return kCostCycle;
   return w >= 16 ? kCostCycle : w == 8 ? 2.0 : w == 4 ? 4.0 : 6.0;
+}
+
+// Widest word (at most 16 bytes) a segmented tile moves: it divides the
+// segments, both row lengths and the unit, or the unit divides it.
+int64_t segtile_word(const Op& o) {
+  int64_t w = pow2_factor(gcd64(gcd64(o.segb, o.C * o.U), o.R * o.U), 16);
+  while (w > 1 && (o.U % w) && (w % o.U)) w /= 2;
+  return w;
+}
+// Segmented tile: a tile pass whose loads and stores are runs of segb bytes.
+inline double segtile_cost(const Op& o) {
+  const int64_t w = segtile_word(o);
+  const double run = o.segb >= 256 ? 1.0 : o.segb >= 128 ? 1.1 : o.segb >= 64 ? 1.5 : 2.5;
+  return run * (w >= 16 ? 1.0 : w == 8 ? 1.5 : 3.0);
 }
 
 struct Choice {
@@ -1000,6 +1016,7 @@ double plan_cost(const Plan& plan) {
       case OP_ROW: c += cat_pass_cost(o.us, o.staged != 0); break;
       case OP_CYCLE: c += cycle_cost(o.U); break;
       case OP_BLOCK: c += kCostBlock; break;
+      case OP_SEGTILE: c += segtile_cost(o); break;
     }
   }
   return c;
@@ -1163,56 +1180,65 @@ double lower_cellperm(Lowerer& L, const std::vector<int64_t>& cd, const std::vec
   return best;
 }
 
+// The X / Y / Z split of one regroup plan and its two cell permutations.
+struct RegroupSplit {
+  std::vector<int> Z;  // canonical dims of Z, fastest first
+  int64_t px = 1, py = 1, pz = 1;
+  std::vector<int64_t> c1, c3;  // cell dims of steps 1 and 3
+  std::vector<int> t1, t3;      // their 1-based perms
+
+  // X = the first i canonical input dims, Y = the first j output dims.
+  bool init(const Regroup& g, int i, int j, bool z_out_order) {
+    const int n = (int)g.q.size();
+    std::vector<int> inX(n, 0), inY(n, 0);
+    for (int a = 0; a < i; ++a) {
+      inX[a] = 1;
+      px *= g.q[a];
+    }
+    for (int k = 0; k < j; ++k) {
+      if (inX[g.p[k]]) return false;
+      inY[g.p[k]] = 1;
+      py *= g.q[g.p[k]];
+    }
+    if (z_out_order) {
+      for (int k = j; k < n; ++k)
+        if (!inX[g.p[k]]) Z.push_back(g.p[k]);
+    } else {
+      for (int a = i; a < n; ++a)
+        if (!inY[a]) Z.push_back(a);
+    }
+    for (int a : Z) pz *= g.q[a];
+    // step 1: cell dims i..n-1 (input order) -> (Y in output order, Z)
+    for (int a = i; a < n; ++a) c1.push_back(g.q[a]);
+    for (int k = 0; k < j; ++k) t1.push_back(g.p[k] - i + 1);
+    for (int a : Z) t1.push_back(a - i + 1);
+    // step 3: cell dims (X in input order, Z) -> output order of the rest
+    std::vector<int> l3;
+    for (int a = 0; a < i; ++a) l3.push_back(a);
+    l3.insert(l3.end(), Z.begin(), Z.end());
+    for (int a : l3) c3.push_back(g.q[a]);
+    for (int k = j; k < n; ++k)
+      t3.push_back((int)(std::find(l3.begin(), l3.end(), g.p[k]) - l3.begin()) + 1);
+    return true;
+  }
+};
+
 // Best plan of the regroup family for (X = the first i canonical input
 // dims, Y = the first j output dims); false when none lowers.
 bool regroup_plan(const Regroup& g, int i, int j, bool z_out_order, size_t elem, Lowerer& L,
                   const DeviceProfile& prof, Plan& plan) {
-  const int n = (int)g.q.size();
-  std::vector<int> inX(n, 0), inY(n, 0);
-  int64_t px = 1, py = 1, pz = 1;
-  for (int a = 0; a < i; ++a) {
-    inX[a] = 1;
-    px *= g.q[a];
-  }
-  for (int k = 0; k < j; ++k) {
-    if (inX[g.p[k]]) return false;
-    inY[g.p[k]] = 1;
-    py *= g.q[g.p[k]];
-  }
-  std::vector<int> Z;
-  if (z_out_order) {
-    for (int k = j; k < n; ++k)
-      if (!inX[g.p[k]]) Z.push_back(g.p[k]);
-  } else {
-    for (int a = i; a < n; ++a)
-      if (!inY[a]) Z.push_back(a);
-  }
-  for (int a : Z) pz *= g.q[a];
-  // step 1: cell dims i..n-1 (input order) -> (Y in output order, Z)
-  std::vector<int64_t> c1;
-  for (int a = i; a < n; ++a) c1.push_back(g.q[a]);
-  std::vector<int> t1;
-  for (int k = 0; k < j; ++k) t1.push_back(g.p[k] - i + 1);
-  for (int a : Z) t1.push_back(a - i + 1);
-  // step 3: cell dims (X in input order, Z) -> output order of the rest
-  std::vector<int> l3;
-  for (int a = 0; a < i; ++a) l3.push_back(a);
-  l3.insert(l3.end(), Z.begin(), Z.end());
-  std::vector<int64_t> c3;
-  for (int a : l3) c3.push_back(g.q[a]);
-  std::vector<int> t3;
-  for (int k = j; k < n; ++k)
-    t3.push_back((int)(std::find(l3.begin(), l3.end(), g.p[k]) - l3.begin()) + 1);
-
+  RegroupSplit sp;
+  if (!sp.init(g, i, j, z_out_order)) return false;
+  const int64_t px = sp.px, py = sp.py, pz = sp.pz;
   Plan out;
   std::vector<Op> o1, o3;
   std::vector<Stage> s1, s3;
-  if (lower_cellperm(L, c1, t1, px * g.E, 0, prof, o1, s1) >= 1e300) return false;
+  if (lower_cellperm(L, sp.c1, sp.t1, px * g.E, 0, prof, o1, s1) >= 1e300) return false;
   std::vector<Op> o2;
   L.emit(pz, py, px, g.E, 0, (int)s1.size(), 0, o2);
   for (const Op& op : o2)
     if (op.kind == 0) return false;
-  if (lower_cellperm(L, c3, t3, py * g.E, (int)s1.size() + 1, prof, o3, s3) >= 1e300)
+  if (lower_cellperm(L, sp.c3, sp.t3, py * g.E, (int)s1.size() + 1, prof, o3, s3) >= 1e300)
     return false;
   const int64_t ue = g.E / (int64_t)elem;  // canonical element in tensor elements
   for (const Stage& s : s1) out.stages.push_back(make_stage(s.P * px * ue, s.X, s.Y, s.S));
@@ -1223,6 +1249,114 @@ bool regroup_plan(const Regroup& g, int i, int j, bool z_out_order, size_t elem,
   out.ops.insert(out.ops.end(), o3.begin(), o3.end());
   fuse_cycles(out.ops, prof);
   fuse_tiles(out.ops, prof);
+  plan.stages = std::move(out.stages);
+  plan.ops = std::move(out.ops);
+  return true;
+}
+
+// The decomposer order with fewer stages for a cell permutation.
+bool fewest_stages(const std::vector<int64_t>& cd, const std::vector<int>& perm1,
+                   std::vector<Stage>& st) {
+  std::vector<Stage> pre, post;
+  if (!decompose(cd.data(), perm1.data(), (int)cd.size(), REORDER_PRE, pre) ||
+      !decompose(cd.data(), perm1.data(), (int)cd.size(), REORDER_POST, post))
+    return false;
+  st = post.size() < pre.size() ? post : pre;
+  return true;
+}
+
+// Two-pass regroup: both cell permutations of the regroup plan composed into
+// one cycle pass over cells of gcd(X bytes, Y bytes), and the X-by-Y
+// matrices transposed where their runs lie.  Tile last: after the cycle pass
+// the runs of a matrix are its output rows, at their output addresses; tile
+// first: the runs are its input rows at their input addresses, and the
+// cycle pass follows.
+bool regroup2_plan(const Regroup& g, int i, int j, bool z_out_order, bool tile_last,
+                   size_t elem, const DeviceProfile& prof, Plan& plan) {
+  if (prof.cycle_min_cell <= 0) return false;
+  RegroupSplit sp;
+  if (!sp.init(g, i, j, z_out_order)) return false;
+  const int n = (int)g.q.size();
+  const int64_t Xb = sp.px * g.E, Yb = sp.py * g.E, gc = gcd64(Xb, Yb);
+  const int64_t fp_max = prof.tile_fp_target > 0 ? prof.tile_fp_target : prof.lds_max / 2;
+  if (gc < prof.cycle_min_cell || sp.px * Yb > fp_max || (int)sp.Z.size() > 6) return false;
+  if ((tile_last ? i : j) > 6 || sp.pz >= (1ll << 31)) return false;
+  {
+    // padded LDS rows and the segment addresses must fit one workgroup
+    const int64_t rs = tile_row_stride(Xb, pow2_factor(g.E, 16));
+    const int64_t nseg = tile_last ? sp.px : sp.py;
+    if ((sp.py * rs + 15) / 16 * 16 + nseg * 8 > prof.lds_max) return false;
+  }
+  std::vector<Stage> s1, s3;
+  if (!fewest_stages(sp.c1, sp.t1, s1) || !fewest_stages(sp.c3, sp.t3, s3)) return false;
+  if (s1.size() + s3.size() > 4) return false;
+  std::vector<CycleMap> maps;
+  int64_t N = 1;
+  for (int side = 0; side < 2; ++side)
+    for (const Stage& st : side ? s3 : s1) {
+      const int64_t f = st.P * (side ? Yb : Xb) / gc;
+      maps.push_back(CycleMap{st.Y * st.X * f, st.Y, st.X, f});
+      N = N / gcd64(N, maps.back().bc) * maps.back().bc;
+    }
+  uint64_t vol = 1;
+  for (int64_t d : g.q) vol *= (uint64_t)d;
+  const uint64_t cells = vol * (uint64_t)g.E / (uint64_t)gc;
+  if ((uint64_t)N > kCycleMaxCells || cells % (uint64_t)N) return false;
+
+  // byte strides of the canonical dims in the input and output layouts
+  std::vector<int64_t> ist(n), ost(n);
+  int64_t acc = g.E;
+  for (int a = 0; a < n; ++a) {
+    ist[a] = acc;
+    acc *= g.q[a];
+  }
+  acc = g.E;
+  for (int k = 0; k < n; ++k) {
+    ost[g.p[k]] = acc;
+    acc *= g.q[g.p[k]];
+  }
+  Op t;
+  t.kind = OP_SEGTILE;
+  t.stage = (int)s1.size();
+  t.B = sp.pz;
+  t.R = sp.py;
+  t.C = sp.px;
+  t.U = g.E;
+  t.us = g.E;
+  t.segb = tile_last ? Yb : Xb;
+  const std::vector<int64_t>& zst = tile_last ? ost : ist;
+  if (tile_last)
+    for (int a = 0; a < i; ++a) {
+      t.sin_radix[t.sin_n] = g.q[a];
+      t.sin_stride[t.sin_n++] = ost[a];
+    }
+  else
+    for (int k = 0; k < j; ++k) {
+      t.sin_radix[t.sin_n] = g.q[g.p[k]];
+      t.sin_stride[t.sin_n++] = ist[g.p[k]];
+    }
+  for (int a : sp.Z) {
+    t.sout_radix[t.sout_n] = g.q[a];
+    t.sout_stride[t.sout_n++] = zst[a];
+  }
+  Plan out;
+  if (!maps.empty()) {
+    Op c;
+    c.kind = OP_CYCLE;
+    c.stage = tile_last ? 0 : (int)s1.size() + 1;
+    c.B = (int64_t)(cells / (uint64_t)N);
+    c.R = N;
+    c.C = 1;
+    c.U = gc;
+    c.us = gc;
+    c.cmaps = maps;
+    out.ops.push_back(c);
+  }
+  out.ops.insert(tile_last ? out.ops.end() : out.ops.begin(), t);
+  const int64_t ue = g.E / (int64_t)elem;
+  for (const Stage& s : s1) out.stages.push_back(make_stage(s.P * sp.px * ue, s.X, s.Y, s.S));
+  out.stages.push_back(make_stage(ue, sp.px, sp.py, sp.pz));
+  for (const Stage& s : s3) out.stages.push_back(make_stage(s.P * sp.py * ue, s.X, s.Y, s.S));
   plan.stages = std::move(out.stages);
   plan.ops = std::move(out.ops);
   return true;
@@ -1404,15 +1538,19 @@ int make_plan(const int64_t* dims_in, const int* perm1, int rank, size_t elem, i
       for (int j = 1; i + j <= n && j <= 6; ++j)
         for (bool zo : {false, true}) {
           if (zo && i + j == n) continue;  // no Z: both orders agree
-          Plan q;
-          if (!regroup_plan(rg, i, j, zo, elem, shared, p2, q)) continue;
-          q.tensor_bytes = vol * elem;
-          const double c = plan_cost(q);
-          if (rc || c < best_cost - 1e-9) {
-            best_cost = c;
-            best_mult = mult;
-            plan = std::move(q);
-            rc = EITHOT_OK;
+          for (int form = 0; form < 3; ++form) {
+            Plan q;
+            const bool ok = form == 0 ? regroup_plan(rg, i, j, zo, elem, shared, p2, q)
+                                      : regroup2_plan(rg, i, j, zo, form == 1, elem, p2, q);
+            if (!ok) continue;
+            q.tensor_bytes = vol * elem;
+            const double c = plan_cost(q);
+            if (rc || c < best_cost - 1e-9) {
+              best_cost = c;
+              best_mult = mult;
+              plan = std::move(q);
+              rc = EITHOT_OK;
+            }
           }
         }
   }
@@ -1447,7 +1585,7 @@ int make_plan(const int64_t* dims_in, const int* perm1, int rank, size_t elem, i
 
 std::string Plan::describe() const {
   std::ostringstream os;
-  static const char* kn[] = {"?", "tile", "col", "row", "cycle", "block"};
+  static const char* kn[] = {"?", "tile", "col", "row", "cycle", "block", "segtile"};
   static const char* mn[] = {"-", "rot", "rotinv", "colsh", "colshinv", "rowg", "rowd"};
   os << "stages:";
   for (const Stage& s : stages) {

@@ -35,7 +35,7 @@ bool decompose(const int64_t* dims, const int* perm1, int rank, int reorder,
                std::vector<Stage>& out);
 
 // ---- device passes ---------------------------------------------------------
-enum OpKind : int { OP_TILE = 1, OP_COL = 2, OP_ROW = 3, OP_CYCLE = 4, OP_BLOCK = 5 };
+enum OpKind : int { OP_TILE = 1, OP_COL = 2, OP_ROW = 3, OP_CYCLE = 4, OP_BLOCK = 5, OP_SEGTILE = 6 };
 
 // Index maps of the passes, all in gather form (dest <- source).
 enum MapKind : int {
@@ -125,6 +125,16 @@ struct Op {
   std::shared_ptr<const CycleTable> cyc;  // OP_CYCLE: segments shared by all B regions
   std::shared_ptr<const BlockTable> blk;  // OP_BLOCK: B blocks of R cells of U bytes
   std::vector<int64_t> bparts;            // OP_BLOCK: the fused tiles, (R, C, U) each
+  // OP_SEGTILE: B matrices of R x C units of U bytes (C fastest), each
+  // transposed in place.  Matrix b's bytes, in row-major order, lie in
+  // R*C*U/segb segments of segb bytes; segment s of matrix b starts at byte
+  // outer(b) + inner(s), where outer/inner sum digit * stride over the
+  // mixed-radix digits (fastest first) of b and of s.  The transposed matrix
+  // is written back over the same segments in the same order.
+  int64_t segb = 0;
+  int sin_n = 0, sout_n = 0;
+  int64_t sin_radix[6] = {0}, sin_stride[6] = {0};
+  int64_t sout_radix[6] = {0}, sout_stride[6] = {0};
 
   int64_t footprint_units() const;   // units staged per footprint
   int64_t footprint_bytes() const { return footprint_units() * us; }

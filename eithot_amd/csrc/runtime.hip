@@ -410,6 +410,33 @@ int eithot_plan_cycle_maps(const int64_t* dims, const int* perm, int order, size
   return (int)mv.size();
 }
 
+int eithot_plan_segments(const int64_t* dims, const int* perm, int order, size_t elem_bytes,
+                         int n_sub, double alpha, int64_t lds_max, int op_index, int64_t* segb,
+                         int64_t* inner, int64_t* outer) {
+  DeviceProfile prof;
+  if (lds_max > 0) prof.lds_max = lds_max;
+  tuning_apply(tuning_get(), prof);
+  Plan plan;
+  int rc = make_plan(dims, perm, order, elem_bytes, n_sub, alpha, prof, prof.reorder, plan);
+  if (rc) return fail(rc, "planning failed (%d)", rc);
+  if (op_index < 0 || op_index >= (int)plan.ops.size() ||
+      plan.ops[op_index].kind != OP_SEGTILE)
+    return fail(EITHOT_ERR_INVALID, "pass %d is not a segmented tile pass", op_index);
+  const Op& o = plan.ops[op_index];
+  if (segb) *segb = o.segb;
+  for (int k = 0; k < 6; ++k) {
+    if (inner) {
+      inner[2 * k] = k < o.sin_n ? o.sin_radix[k] : 1;
+      inner[2 * k + 1] = k < o.sin_n ? o.sin_stride[k] : 0;
+    }
+    if (outer) {
+      outer[2 * k] = k < o.sout_n ? o.sout_radix[k] : 1;
+      outer[2 * k + 1] = k < o.sout_n ? o.sout_stride[k] : 0;
+    }
+  }
+  return o.sin_n + 16 * o.sout_n;
+}
+
 int eithot_set_tuning(const char* key, int64_t value) {
   int rc = tuning_set(key, value);
   if (rc) return fail(rc, "unknown tuning key or value out of range");

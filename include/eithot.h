@@ -111,7 +111,8 @@ int eithot_decompose(const int64_t* dims, const int* perm, int order, int reorde
 /* One device pass of a plan, for introspection and tests. */
 typedef struct {
   int kind;      /* 1 tile, 2 column, 3 row, 4 tile by cycle following,
-                    5 fused block permutation (B blocks of R cells of U bytes) */
+                    5 fused block permutation (B blocks of R cells of U bytes),
+                    6 segmented tile (see eithot_plan_segments) */
   int map;       /* 0 none, 1 rot, 2 rotinv, 3 colsh, 4 colshinv, 5 rowg, 6 rowd */
   int staged;    /* 1 when executed through scratch */
   int stage;     /* decomposer stage, -1 for partition/join */
@@ -142,6 +143,18 @@ int64_t eithot_plan_block_table(const int64_t* dims, const int* perm, int order,
 int eithot_plan_cycle_maps(const int64_t* dims, const int* perm, int order, size_t elem_bytes,
                            int n_sub, double alpha, int64_t lds_max, int op_index,
                            int64_t* maps, int max_maps);
+
+/* Segments of pass op_index when it is a segmented tile pass (kind 6):
+ * B matrices of R x C units of U bytes, each transposed in place, whose
+ * bytes (row-major) lie in segments of *segb bytes; segment s of matrix b
+ * starts at byte outer(b) + inner(s) of the tensor, where inner(s) sums
+ * digit * stride over the mixed-radix digits of s (fastest first) and
+ * outer(b) the same over b.  inner and outer receive 6 (radix, stride)
+ * pairs each (unused: radix 1).  Returns n_inner + 16 * n_outer, or a
+ * negative EITHOT_ERR_*.  No counterpart in the reference. */
+int eithot_plan_segments(const int64_t* dims, const int* perm, int order, size_t elem_bytes,
+                         int n_sub, double alpha, int64_t lds_max, int op_index, int64_t* segb,
+                         int64_t* inner, int64_t* outer);
 
 /* ---- tuning -------------------------------------------------------------------
  * No counterpart in the reference, whose equivalents are compile-time

@@ -26,6 +26,27 @@ def simulate(buf, ops):
             view[o["base"]:o["base"] + nb] = blk[:, o["table"], :].reshape(-1)
             i += 1
             continue
+        if o["kind"] == 6:  # segmented tile: gather each matrix's segments, transpose, scatter
+            B, R, C, U, sb = o["B"], o["R"], o["C"], o["U"], o["segb"]
+            nseg = R * C * U // sb
+
+            def digits_sum(count, digits):
+                idx = np.arange(count, dtype=np.int64)
+                out = np.zeros(count, dtype=np.int64)
+                for radix, stride in digits:
+                    out += (idx % radix) * stride
+                    idx //= radix
+                assert not idx.any(), "index beyond its digits"
+                return out
+
+            addr = (digits_sum(B, o["outer"])[:, None, None] +
+                    digits_sum(nseg, o["inner"])[None, :, None] +
+                    np.arange(sb, dtype=np.int64)[None, None, :]).reshape(B, -1)
+            assert len(np.unique(addr)) == addr.size, "segments overlap"
+            m = view[addr].reshape(B, R, C, U)
+            view[addr] = m.transpose(0, 2, 1, 3).reshape(B, -1)
+            i += 1
+            continue
         if o["kind"] == 4 and "maps" in o:  # cycle following: its cell maps in pass order
             nb = o["B"] * o["R"] * o["C"] * o["U"]
             blk = np.ascontiguousarray(view[o["base"]:o["base"] + nb])

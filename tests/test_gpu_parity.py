@@ -171,3 +171,37 @@ def test_random_high_order(torch_cuda):
                                    alpha=0.5)
         assert_bitexact(got, want, f"{dims} {perm} elem {elem}")
         assert peak <= 0.5 * got.nbytes
+
+
+SEGTILE_CASES = [([12, 12, 12, 16, 20], [1, 5, 3, 2, 4], 4), ([2, 32, 4, 5, 32], [5, 2, 4, 1, 3], 4),
+                 ([8, 8, 10, 16, 16, 3], [1, 5, 3, 2, 4, 6], 1), ([48, 32, 64], [3, 2, 1], 8),
+                 ([24, 40, 24, 24], [3, 2, 1, 4], 8), ([2, 24, 4, 8, 16, 16], [4, 6, 5, 3, 2, 1], 2),
+                 ([16, 24, 5, 16, 16], [5, 4, 2, 1, 3], 8), ([3, 10, 8, 12, 8, 16], [6, 4, 3, 1, 2, 5], 8),
+                 # the benchmark's two directions at a ninth of its size
+                 ([32, 48, 16, 40, 9, 8], [6, 4, 2, 5, 1, 3], 4),
+                 ([8, 40, 48, 9, 32, 16], [5, 3, 6, 2, 4, 1], 4)]
+
+
+def test_segtile_plans(torch_cuda):
+    # a cycle pass composed of both cell permutations plus a segmented tile
+    # (tile first and tile last; 1- to 8-byte units; 16-byte words), and the
+    # same tensors at a data pointer offset by one unit (narrower words)
+    import eithot_amd as E
+    for dims, perm, elem in SEGTILE_CASES:
+        ops, _ = E.plan_describe(dims, perm, elem, 1, 0.05)
+        assert any(o["kind"] == 6 for o in ops), (dims, perm, ops)
+        got, want, peak = run_case(torch_cuda, dims, perm, elem, seed=14, alpha=0.05)
+        assert_bitexact(got, want, f"segtile {dims} {perm} elem {elem}")
+        assert peak <= max(0.05 * got.nbytes, 1 << 20)
+    from oracle import pyoracle as O
+    for dims, perm, elem in SEGTILE_CASES[::3]:
+        vol = int(np.prod(dims))
+        host = O.fill(vol + 1, elem, 15, 1)
+        dev = torch_cuda.from_numpy(host.copy()).to("cuda:0")
+        print(f"case offset 1 dims={dims} perm={perm} elem={elem}", flush=True)
+        E.transpose(dev, 1, len(dims), dims, [p - 1 for p in perm], elem, 1, 0.05)
+        torch_cuda.cuda.synchronize()
+        got = dev.cpu().numpy()
+        want = host.copy()
+        want[elem:] = O.permute(np.ascontiguousarray(host[elem:]), dims, [p - 1 for p in perm], elem)
+        assert_bitexact(got, want, f"segtile offset {dims} {perm} elem {elem}")
