@@ -1910,10 +1910,10 @@ hipError_t launch_segtile(const Op& op, char* data, const LaunchCfg& cfg, hipStr
 //   70 + log2(word)             cycle following (item: blockDim threads)
 //   80 + log2(word)*2 + vec     fused block permutation (item: K blocks)
 //   90 + log2(word)             boundary save of a cycle pass (item: blockDim words)
+//   100 + log2(word)*5 + log2(V) segmented tile (item: K matrices)
+constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
 constexpr size_t kBatchArgBytes =
-    sizeof(Args<uint32_t>) > sizeof(CycArgs)
-        ? (sizeof(Args<uint32_t>) > sizeof(BlkArgs) ? sizeof(Args<uint32_t>) : sizeof(BlkArgs))
-        : (sizeof(CycArgs) > sizeof(BlkArgs) ? sizeof(CycArgs) : sizeof(BlkArgs));
+    cmax(cmax(sizeof(Args<uint32_t>), sizeof(CycArgs)), cmax(sizeof(BlkArgs), sizeof(SegArgs)));
 struct BatchOpDev {
   int variant;
   int pad[3];
@@ -1965,6 +1965,12 @@ __device__ __forceinline__ void batch_body(const BatchOpDev& op, uint32_t f0, un
         *reinterpret_cast<const BlkArgs*>(op.raw), item.f0, lds);                           \
     break;
 
+#define EH_SEG(WL, VL)                                                                      \
+  case 100 + WL * 5 + VL:                                                                   \
+    seg_body<typename WordOf<WL>::T, (1 << VL)>(*reinterpret_cast<const SegArgs*>(op.raw),  \
+                                                item.f0, lds);                              \
+    break;
+
 __global__ __launch_bounds__(512) void k_batch(const BatchOpDev* __restrict__ ops,
                                                const BatchItemDev* __restrict__ items,
                                                uint32_t nitems) {
@@ -1994,12 +2000,28 @@ __global__ __launch_bounds__(512) void k_batch(const BatchOpDev* __restrict__ op
       EH_BLK(3, 0)
       EH_BLK(3, 1)
       EH_BLK(4, 0)
+      EH_SEG(0, 0)
+      EH_SEG(1, 0)
+      EH_SEG(1, 1)
+      EH_SEG(2, 0)
+      EH_SEG(2, 1)
+      EH_SEG(2, 2)
+      EH_SEG(3, 0)
+      EH_SEG(3, 1)
+      EH_SEG(3, 2)
+      EH_SEG(3, 3)
+      EH_SEG(4, 0)
+      EH_SEG(4, 1)
+      EH_SEG(4, 2)
+      EH_SEG(4, 3)
+      EH_SEG(4, 4)
       default: break;
     }
     __syncthreads();  // the next item reuses the LDS
   }
 }
 #undef EH_BLK
+#undef EH_SEG
 #undef EH_CYC
 #undef EH_CASES
 #undef EH_CASES_W
@@ -2067,6 +2089,11 @@ int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, cons
       int64_t w, lds, groups;
       int vec;
       if (prepare_block_args(op, pr.second, cfg, a, &w, &vec, &lds, &groups) == hipSuccess)
+        big = big || lds > 96 * 1024;
+    } else if (op.kind == OP_SEGTILE) {
+      SegArgs a;
+      int64_t w, V, lds, groups;
+      if (prepare_seg_args(op, pr.second, cfg, a, &w, &V, &lds, &groups) == hipSuccess)
         big = big || lds > 96 * 1024;
     } else if (!op.staged && op.kind >= OP_TILE && op.kind <= OP_ROW && !needs64(op)) {
       Args<uint32_t> a;
@@ -2136,6 +2163,22 @@ int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, cons
       lds = std::max(lds, blds);
       for (int64_t g = 0; g < groups; ++g)
         items.push_back({blds, BatchItemDev{idx, (uint32_t)(g * a.K)}});
+      continue;
+    }
+    if (op.kind == OP_SEGTILE) {
+      SegArgs a;
+      int64_t w = 1, V = 1, slds = 0, groups = 0;
+      if (prepare_seg_args(op, data, cfg, a, &w, &V, &slds, &groups) != hipSuccess) {
+        reject(i);
+        continue;
+      }
+      std::memcpy(d.raw, &a, sizeof(a));
+      d.variant = 100 + log2i(w) * 5 + log2i(V);
+      const uint32_t idx = (uint32_t)hops.size();
+      hops.push_back(d);
+      lds = std::max(lds, slds);
+      for (int64_t g = 0; g < groups; ++g)
+        items.push_back({slds, BatchItemDev{idx, (uint32_t)(g * a.K)}});
       continue;
     }
     if (op.staged || op.kind < OP_TILE || op.kind > OP_ROW || needs64(op)) {

@@ -38,11 +38,13 @@ def test_batch_matches_oracle(torch_cuda):
     # maps) and fused block passes, which run inside the batched launches
     specs += [([128, 8, 36, 5], [1, 3, 2, 4], 4), ([1536, 64, 12], [1, 3, 2], 4),
               ([48, 40, 8, 30], [3, 2, 1, 4], 4), ([12, 2, 25, 14, 53, 3], [3, 6, 4, 1, 5, 2], 8),
-              ([2, 42, 2, 11, 11], [5, 2, 1, 4, 3], 8), ([32, 40, 48, 24], [1, 3, 2, 4], 4)]
+              ([2, 42, 2, 11, 11], [5, 2, 1, 4, 3], 8), ([32, 40, 48, 24], [1, 3, 2, 4], 4),
+              # segmented tiles, tile last and tile first
+              ([12, 12, 12, 16, 20], [1, 5, 3, 2, 4], 4), ([2, 32, 4, 5, 32], [5, 2, 4, 1, 3], 4)]
     kinds = set()
     for dims, perm, elem in specs:
         kinds |= {o["kind"] for o in E.plan_describe(dims, perm, elem, 1, 0.5)[0]}
-    assert {4, 5} <= kinds, kinds
+    assert {4, 5, 6} <= kinds, kinds
     buf, offs = carve(torch, specs)
     hosts = []
     for k, ((dims, perm, elem), off) in enumerate(zip(specs, offs)):
