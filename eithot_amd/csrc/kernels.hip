@@ -1158,21 +1158,6 @@ __global__ __launch_bounds__(256) void k_cycle(CycArgs a) {
   cycle_body<Word, D, NT>(a, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// Boundary save of a cycle pass, one word per thread: gtid indexes the
-// words of all B * nslots saved cells.
-template <typename Word>
-__device__ __forceinline__ void cycle_save_word(const CycArgs& a, const uint64_t t) {
-  constexpr uint64_t w = sizeof(Word);
-  const uint64_t wpc = a.U / w;
-  if (t >= a.B * a.nslots * wpc) return;
-  const uint64_t cell = t / wpc;  // boundary cell = b * nslots + slot
-  const uint64_t x = t - cell * wpc;
-  const uint64_t b = cell / a.nslots;
-  const uint64_t sl = cell - b * a.nslots;
-  *reinterpret_cast<Word*>(a.boundary + cell * a.U + x * w) = *reinterpret_cast<const Word*>(
-      a.data + a.base + b * a.RCU + (uint64_t)a.slots[sl] * a.U + x * w);
-}
-
 // Pipelined variant: every lane moves V words per position (a wave covers
 // V*64 words of the cell, each load one coalesced row of the wave), and the
 // loads of the next D positions are issued before the stores of the current
@@ -1283,7 +1268,7 @@ hipError_t launch_cycle_w(CycArgs a, int variant, hipStream_t stream) {
     hipLaunchKernelGGL((k_cycle_save<Word>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
   }
   const uint64_t blocks = ((a.items << a.lane_log2) + 255) / 256;
-  if (a.lane_log2 < 6)  // small cells: several items per wave
+  if (a.lane_log2 < 6 && variant >= 1 && variant <= 3)  // one item per wave only
     hipLaunchKernelGGL((k_cycle<Word, 8>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
   else if (variant == 1)
     hipLaunchKernelGGL((k_cycle_pipe<Word, 4, 2>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
@@ -1306,6 +1291,8 @@ hipError_t launch_cycle_w(CycArgs a, int variant, hipStream_t stream) {
 
 // lanes' words per position of each cycle kernel variant
 inline int cycle_vwords(int variant) { return variant == 1 ? 2 : variant == 3 ? 4 : 1; }
+// the variants that take several items per wave (cells under 64 words)
+inline bool cycle_small_ok(int variant) { return variant == 0 || variant >= 4; }
 
 // Arguments of a cycle pass over the tensor at `data` with its boundary
 // cells in `scratch`; *w receives the word size, *variant the kernel shape
@@ -1350,7 +1337,7 @@ hipError_t prepare_cycle_args(const Op& op, char* data, char* scratch, int64_t s
   // cells smaller than a wave's row of words: 2^lane_log2 lanes per cell
   a.lane_log2 = 6;
   while (a.lane_log2 > 0 && ((uint64_t)w << (a.lane_log2 - 1)) >= (uint64_t)op.U) --a.lane_log2;
-  if (a.lane_log2 < 6) *variant = 0;
+  if (a.lane_log2 < 6 && !cycle_small_ok(*variant)) *variant = 0;
   a.chunk_bytes = (uint32_t)((w << a.lane_log2) * cycle_vwords(*variant));
   a.B = op.B;
   a.chunks = (op.U + a.chunk_bytes - 1) / a.chunk_bytes;

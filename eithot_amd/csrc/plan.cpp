@@ -471,12 +471,13 @@ const std::vector<int64_t>& divisors(int64_t x) {
 // short enough for about 32 Ki waves (one per segment and 1 KiB of cell),
 // long enough that the saved boundary cells and the table fit the budget.
 std::shared_ptr<CycleTable> choose_cycle_table(const std::vector<CycleMap>& maps, int64_t N,
-                                               int64_t B, int64_t U, int64_t budget) {
+                                               int64_t B, int64_t U, int64_t budget,
+                                               int64_t lmax0) {
   const int64_t chunk = 64 * pow2_factor(U, 16);
   const int64_t chunks = (U + chunk - 1) / chunk;
-  int64_t lmax = 256;
+  int64_t lmax = lmax0 > 0 ? lmax0 : 256;
   std::shared_ptr<CycleTable> t = build_cycle_table_maps(maps, N, lmax);
-  while (lmax > 16 && B * (int64_t)t->segs.size() * chunks < 32768) {
+  while (!lmax0 && lmax > 16 && B * (int64_t)t->segs.size() * chunks < 32768) {
     lmax /= 2;
     t = build_cycle_table_maps(maps, N, lmax);
   }
@@ -773,6 +774,7 @@ static int64_t* tuning_field(Tuning& t, const char* key) {
   if (!strcmp(key, "cycle_variant")) return &t.cycle_variant;
   if (!strcmp(key, "reorder")) return &t.reorder;
   if (!strcmp(key, "regroup")) return &t.regroup;
+  if (!strcmp(key, "cycle_lmax")) return &t.cycle_lmax;
   return nullptr;
 }
 
@@ -785,6 +787,7 @@ int tuning_set(const char* key, int64_t value) {
   if (threads && (value < 64 || value > 512 || value % 64)) return EITHOT_ERR_INVALID;
   if (f == &t.reorder && value > 3) return EITHOT_ERR_INVALID;
   if (f == &t.cycle_variant && value > 7) return EITHOT_ERR_INVALID;
+  if (f == &t.cycle_lmax && value && (value < 2 || value > (1ll << 20))) return EITHOT_ERR_INVALID;
   if (f == &t.lds_target && value < 1024) return EITHOT_ERR_INVALID;
   *f = value;
   t.epoch = g_tune.epoch + 1;
@@ -812,6 +815,7 @@ void tuning_apply(const Tuning& t, DeviceProfile& prof) {
   prof.cycle_word_cost = t.cycle_word_cost;
   prof.reorder = (int)t.reorder;
   prof.regroup = t.regroup;
+  prof.cycle_lmax = t.cycle_lmax;
 }
 
 void lower_transpose2d(int64_t B, int64_t R, int64_t C, int64_t U, uint64_t base, int stage,
@@ -1450,7 +1454,7 @@ static void finalize_plan(Plan& plan, double alpha, int64_t reserve, const Devic
     if (op.kind == OP_BLOCK && !op.blk)
       op.blk = compose_tiles(op.bparts, op.R * op.U, op.U, prof.block_digits != 0);
     if (op.kind == OP_CYCLE && !op.cyc) {
-      op.cyc = choose_cycle_table(op.cmaps, op.R * op.C, op.B, op.U, budget);
+      op.cyc = choose_cycle_table(op.cmaps, op.R * op.C, op.B, op.U, budget, prof.cycle_lmax);
     }
     plan.scratch_bytes = std::max(plan.scratch_bytes, staged_scratch(op, budget, prof));
     if (op.cyc) plan.table_bytes += op.cyc->bytes();

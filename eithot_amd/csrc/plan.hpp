@@ -159,6 +159,7 @@ struct DeviceProfile {
   int64_t cycle_word_cost = 1;    // cycle passes of cells in narrow words cost more
   int reorder = 0;                // 0 cheapest of pre/post (mix breaks ties), 1 pre, 2 post, 3 mix
   int64_t regroup = 1;            // with reorder 0: also plan cell pass / tile / cell pass
+  int64_t cycle_lmax = 0;         // > 0: cycle segments of at most this many positions
   double staged_mult = 1.0;       // planner-internal: weight of staged passes in the search
 };
 
@@ -179,6 +180,7 @@ struct Tuning {
   int64_t cycle_variant = 0;
   int64_t reorder = 0;
   int64_t regroup = 1;
+  int64_t cycle_lmax = 0;
   uint64_t epoch = 0;  // bumped on every change (plan caches key on it)
 };
 Tuning tuning_get();

@@ -172,7 +172,10 @@ int eithot_plan_segments(const int64_t* dims, const int* perm, int order, size_t
  *   "fuse_max_block" largest block of a fused tile permutation (65536; 0 = off)
  *   "block_digits"   fused blocks compute source cells from digits (1) or read
  *                    a gather table (0)
- *   "cycle_variant"  cycle-following kernel shape (0..7; 0 default)
+ *   "cycle_variant"  cycle-following kernel shape (0..7; 0 default; 1..3 only
+ *                    for cells of at least 64 words)
+ *   "cycle_lmax"     > 0: cycle segments of at most this many positions
+ *                    (0: chosen per pass)
  *   "cycle_word_cost" cycle passes over cells that are not whole 16-byte
  *                    words cost more in the planner (1) or the same (0)
  *   "threads", "threads_big"  workgroup sizes (256, 512; multiples of 64, at most 512)

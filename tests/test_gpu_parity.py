@@ -72,7 +72,8 @@ def test_cycle_following_passes(torch_cuda):
              ([999, 17, 29], [1, 3, 2], 1),           # odd cells: 1-byte words
              ([257, 40, 9], [1, 3, 2], 2),            # 514-byte cells, 2-byte words
              ([300, 40, 7, 2], [1, 3, 2, 4], 2),      # 600-byte cells, 8-byte words
-             ([129, 33, 31], [1, 3, 2], 4)]           # 516-byte cells, 4-byte words
+             ([129, 33, 31], [1, 3, 2], 4),           # 516-byte cells, 4-byte words
+             ([32, 150, 140, 3], [1, 3, 2, 4], 4)]    # 128-byte cells: 8 items per wave
     default = E.get_tuning("cycle_variant")
     E.set_tuning("cycle_word_cost", 0)  # cells of odd sizes take cycle passes too
     try:
