@@ -1158,6 +1158,21 @@ __global__ __launch_bounds__(256) void k_cycle(CycArgs a) {
   cycle_body<Word, D, NT>(a, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// Boundary save of a cycle pass, one word per thread: gtid indexes the
+// words of all B * nslots saved cells.
+template <typename Word>
+__device__ __forceinline__ void cycle_save_word(const CycArgs& a, const uint64_t t) {
+  constexpr uint64_t w = sizeof(Word);
+  const uint64_t wpc = a.U / w;
+  if (t >= a.B * a.nslots * wpc) return;
+  const uint64_t cell = t / wpc;  // boundary cell = b * nslots + slot
+  const uint64_t x = t - cell * wpc;
+  const uint64_t b = cell / a.nslots;
+  const uint64_t sl = cell - b * a.nslots;
+  *reinterpret_cast<Word*>(a.boundary + cell * a.U + x * w) = *reinterpret_cast<const Word*>(
+      a.data + a.base + b * a.RCU + (uint64_t)a.slots[sl] * a.U + x * w);
+}
+
 // Pipelined variant: every lane moves V words per position (a wave covers
 // V*64 words of the cell, each load one coalesced row of the wave), and the
 // loads of the next D positions are issued before the stores of the current
