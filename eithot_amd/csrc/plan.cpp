@@ -522,6 +522,10 @@ inline double staged_tile_cost(int64_t R, int64_t U, int64_t us) {
 // Cycle-following moves whole cells (of at least cycle_min_cell bytes), one
 // read and one write each; smaller cells waste HBM bursts.
 constexpr uint64_t kCycleMaxCells = 1ull << 26;
+// Composite cycle tables of two-pass regroup plans may span more cells: the
+// plan replaces a whole pass of a large tensor (host walk: about 40 ns per
+// cell, once per shape).
+constexpr uint64_t kCycleMaxCellsRegroup = 1ull << 28;
 constexpr double kCostCycle = 1.15;  // 4.6-5.8 TB/s for cells of 128 B to 384 KB, 16-byte words
 constexpr double kCostBlock = 1.1;   // fused block permutation
 // Cells whose size is not a multiple of 16 bytes move in narrower words
@@ -1305,7 +1309,7 @@ bool regroup2_plan(const Regroup& g, int i, int j, bool z_out_order, bool tile_l
   uint64_t vol = 1;
   for (int64_t d : g.q) vol *= (uint64_t)d;
   const uint64_t cells = vol * (uint64_t)g.E / (uint64_t)gc;
-  if ((uint64_t)N > kCycleMaxCells || cells % (uint64_t)N) return false;
+  if ((uint64_t)N > kCycleMaxCellsRegroup || cells % (uint64_t)N) return false;
 
   // byte strides of the canonical dims in the input and output layouts
   std::vector<int64_t> ist(n), ost(n);
@@ -1364,6 +1368,79 @@ bool regroup2_plan(const Regroup& g, int i, int j, bool z_out_order, bool tile_l
   plan.stages = std::move(out.stages);
   plan.ops = std::move(out.ops);
   return true;
+}
+
+// g with canonical dim a split into (f, q[a] / f), the fast factor first in
+// both orders.
+Regroup split_dim(const Regroup& g, int a, int64_t f) {
+  Regroup h;
+  h.E = g.E;
+  for (int x = 0; x < (int)g.q.size(); ++x) {
+    if (x == a) {
+      h.q.push_back(f);
+      h.q.push_back(g.q[a] / f);
+    } else {
+      h.q.push_back(g.q[x]);
+    }
+  }
+  for (int v : g.p) {
+    if (v == a) {
+      h.p.push_back(a);
+      h.p.push_back(a + 1);
+    } else {
+      h.p.push_back(v > a ? v + 1 : v);
+    }
+  }
+  return h;
+}
+
+// Two-pass regroup plans whose X-by-Y matrix is too large for LDS: the
+// last dim of X and the last dim of Y are split, and X (Y) takes only their
+// fast factors.  The splits giving the largest common cell, then the
+// largest matrix, are tried.  best_cost: the cost to beat; on success the
+// cheaper plan is in `plan`.
+bool regroup2_split(const Regroup& g, int i, int j, bool z_out_order, bool tile_last,
+                    size_t elem, const DeviceProfile& prof, double best_cost, Plan& plan) {
+  if (prof.cycle_min_cell <= 0) return false;
+  RegroupSplit sp;
+  if (!sp.init(g, i, j, z_out_order)) return false;
+  const int64_t fp_max = prof.tile_fp_target > 0 ? prof.tile_fp_target : prof.lds_max / 2;
+  if (sp.px * sp.py * g.E <= fp_max) return false;  // no split needed
+  const int a = i - 1, b = g.p[j - 1];
+  const int64_t xo = sp.px / g.q[a], yo = sp.py / g.q[b];  // X, Y without their last dims
+  struct Cand {
+    int64_t f, h, gc, tile;
+  };
+  std::vector<Cand> cands;
+  for (int64_t f : divisors(g.q[a])) {
+    const int64_t Xb = xo * f * g.E;
+    if (Xb < prof.cycle_min_cell || Xb * yo > fp_max) continue;
+    for (int64_t h : divisors(g.q[b])) {
+      const int64_t Yb = yo * h * g.E;
+      if (Xb * (Yb / g.E) > fp_max) break;  // divisors ascend
+      const int64_t gc = gcd64(Xb, Yb);
+      if (gc < prof.cycle_min_cell || (f == g.q[a] && h == g.q[b])) continue;
+      cands.push_back(Cand{f, h, gc, Xb * (Yb / g.E)});
+    }
+  }
+  std::sort(cands.begin(), cands.end(), [](const Cand& x, const Cand& y) {
+    return x.gc != y.gc ? x.gc > y.gc : x.tile > y.tile;
+  });
+  bool found = false;
+  for (size_t c = 0; c < cands.size() && c < 4; ++c) {
+    Regroup g1 = cands[c].f < g.q[a] ? split_dim(g, a, cands[c].f) : g;
+    const int b1 = g1.p[j - 1];
+    Regroup g2 = cands[c].h < g1.q[b1] ? split_dim(g1, b1, cands[c].h) : g1;
+    Plan q;
+    if (!regroup2_plan(g2, i, j, z_out_order, tile_last, elem, prof, q)) continue;
+    const double cost = plan_cost(q);
+    if (cost < best_cost - 1e-9) {
+      best_cost = cost;
+      plan = std::move(q);
+      found = true;
+    }
+  }
+  return found;
 }
 
 }  // namespace
@@ -1542,10 +1619,13 @@ int make_plan(const int64_t* dims_in, const int* perm1, int rank, size_t elem, i
       for (int j = 1; i + j <= n && j <= 6; ++j)
         for (bool zo : {false, true}) {
           if (zo && i + j == n) continue;  // no Z: both orders agree
-          for (int form = 0; form < 3; ++form) {
+          for (int form = 0; form < 5; ++form) {
             Plan q;
-            const bool ok = form == 0 ? regroup_plan(rg, i, j, zo, elem, shared, p2, q)
-                                      : regroup2_plan(rg, i, j, zo, form == 1, elem, p2, q);
+            const bool ok =
+                form == 0   ? regroup_plan(rg, i, j, zo, elem, shared, p2, q)
+                : form <= 2 ? regroup2_plan(rg, i, j, zo, form == 1, elem, p2, q)
+                            : regroup2_split(rg, i, j, zo, form == 3, elem, p2,
+                                             rc ? 1e300 : best_cost, q);
             if (!ok) continue;
             q.tensor_bytes = vol * elem;
             const double c = plan_cost(q);

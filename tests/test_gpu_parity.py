@@ -178,6 +178,9 @@ SEGTILE_CASES = [([12, 12, 12, 16, 20], [1, 5, 3, 2, 4], 4), ([2, 32, 4, 5, 32],
                  ([8, 8, 10, 16, 16, 3], [1, 5, 3, 2, 4, 6], 1), ([48, 32, 64], [3, 2, 1], 8),
                  ([24, 40, 24, 24], [3, 2, 1, 4], 8), ([2, 24, 4, 8, 16, 16], [4, 6, 5, 3, 2, 1], 2),
                  ([16, 24, 5, 16, 16], [5, 4, 2, 1, 3], 8), ([3, 10, 8, 12, 8, 16], [6, 4, 3, 1, 2, 5], 8),
+                 # X and Y split to fit LDS (fast factors of their last dims)
+                 ([160, 64, 160], [3, 2, 1], 8), ([96, 128, 128], [3, 2, 1], 4),
+                 ([48, 96, 2, 96], [4, 2, 3, 1], 2),
                  # the benchmark's two directions at a ninth of its size
                  ([32, 48, 16, 40, 9, 8], [6, 4, 2, 5, 1, 3], 4),
                  ([8, 40, 48, 9, 32, 16], [5, 3, 6, 2, 4, 1], 4)]
