@@ -174,8 +174,9 @@ def test_random_high_order(torch_cuda):
         assert peak <= 0.5 * got.nbytes
 
 
-SEGTILE_CASES = [([12, 12, 12, 16, 20], [1, 5, 3, 2, 4], 4), ([2, 32, 4, 5, 32], [5, 2, 4, 1, 3], 4),
-                 ([8, 8, 10, 16, 16, 3], [1, 5, 3, 2, 4, 6], 1), ([48, 32, 64], [3, 2, 1], 8),
+SEGTILE_CASES = [([14, 12, 14, 9, 9, 8], [5, 3, 2, 4, 1, 6], 8), ([2, 32, 4, 5, 32], [5, 2, 4, 1, 3], 4),
+                 ([8, 12, 16, 16, 16], [5, 4, 1, 3, 2], 1), ([48, 32, 64], [3, 2, 1], 8),
+                 ([10, 48, 64, 5, 8], [4, 3, 5, 1, 2], 1),
                  ([24, 40, 24, 24], [3, 2, 1, 4], 8), ([2, 24, 4, 8, 16, 16], [4, 6, 5, 3, 2, 1], 2),
                  ([16, 24, 5, 16, 16], [5, 4, 2, 1, 3], 8), ([3, 10, 8, 12, 8, 16], [6, 4, 3, 1, 2, 5], 8),
                  # X and Y split to fit LDS (fast factors of their last dims)

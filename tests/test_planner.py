@@ -186,9 +186,9 @@ def test_regroup_plans_execute_correctly():
 COMPOSITE_CASES = [([3, 14, 2, 15, 58, 20], [5, 2, 6, 1, 3, 4], 4),
                    ([12, 2, 25, 14, 53, 3], [3, 6, 4, 1, 5, 2], 8),
                    ([12, 9, 14, 28, 53], [2, 1, 5, 4, 3], 8),
-                   ([2, 3, 2, 11, 3, 39], [4, 3, 2, 6, 1, 5], 8),
                    ([51, 6, 10, 10, 4], [1, 3, 2, 5, 4], 8),
-                   ([2, 39, 6, 2, 6, 13], [4, 5, 6, 3, 1, 2], 4)]
+                   ([2, 39, 6, 2, 6, 13], [4, 5, 6, 3, 1, 2], 4),
+                   ([46, 20, 52, 26], [4, 2, 3, 1], 8)]
 
 
 @pytest.mark.parametrize("dims,perm1,elem", COMPOSITE_CASES)
