@@ -28,13 +28,37 @@ namespace {
 
 constexpr int UNR = 8;  // global loads in flight per thread
 
-struct alignas(16) W16 {
-  uint32_t v[4];
-};
+// 16-byte word: a native vector, so arrays of it stay in registers
+typedef uint32_t W16 __attribute__((ext_vector_type(4)));
 
+// V words packed into one wider store (the union keeps them in registers)
 template <int BYTES>
-struct alignas(BYTES) Vec {
-  unsigned char b[BYTES];
+struct PackOf;
+template <>
+struct PackOf<1> {
+  using T = uint8_t;
+};
+template <>
+struct PackOf<2> {
+  using T = uint16_t;
+};
+template <>
+struct PackOf<4> {
+  using T = uint32_t;
+};
+template <>
+struct PackOf<8> {
+  using T = uint64_t;
+};
+template <>
+struct PackOf<16> {
+  using T = W16;
+};
+template <typename Word, int V>
+union Packed {
+  using P = typename PackOf<V * sizeof(Word)>::T;
+  Word e[V];
+  P p;
 };
 
 __device__ __forceinline__ uint32_t xcd_logical(uint32_t bid, uint32_t n, int remap) {
@@ -54,14 +78,14 @@ __device__ __forceinline__ void lds_put16(unsigned char* p, const W16& v) {
   if ((a & 15) == 0) {
     *reinterpret_cast<W16*>(p) = v;
   } else if ((a & 7) == 0) {
-    reinterpret_cast<uint64_t*>(p)[0] = (uint64_t)v.v[0] | ((uint64_t)v.v[1] << 32);
-    reinterpret_cast<uint64_t*>(p)[1] = (uint64_t)v.v[2] | ((uint64_t)v.v[3] << 32);
+    reinterpret_cast<uint64_t*>(p)[0] = (uint64_t)v[0] | ((uint64_t)v[1] << 32);
+    reinterpret_cast<uint64_t*>(p)[1] = (uint64_t)v[2] | ((uint64_t)v[3] << 32);
   } else {
     uint32_t* q = reinterpret_cast<uint32_t*>(p);
-    q[0] = v.v[0];
-    q[1] = v.v[1];
-    q[2] = v.v[2];
-    q[3] = v.v[3];
+    q[0] = v[0];
+    q[1] = v[1];
+    q[2] = v[2];
+    q[3] = v[3];
   }
 }
 
@@ -238,13 +262,12 @@ __device__ __forceinline__ void tile_body(const Args<Idx>& a, const Idx f0, unsi
       const Idx rem = t - k * a.st_per;
       const Idx j = a.fvpr.div(rem);
       const Idx i0 = (rem - j * a.vpr) * VEC;
-      Word tmp[VEC];
+      Packed<Word, VEC> tmp;
       const unsigned char* l = lds + k * a.fpl + i0 * a.rs + j * w;
 #pragma unroll
-      for (int v = 0; v < VEC; ++v) tmp[v] = *reinterpret_cast<const Word*>(l + v * a.rs);
-      *reinterpret_cast<Vec<VEC * sizeof(Word)>*>(gbase + (uint64_t)k * a.RCU +
-                                                  ((uint64_t)j * a.R + i0) * w) =
-          *reinterpret_cast<const Vec<VEC * sizeof(Word)>*>(tmp);
+      for (int v = 0; v < VEC; ++v) tmp.e[v] = *reinterpret_cast<const Word*>(l + v * a.rs);
+      *reinterpret_cast<typename Packed<Word, VEC>::P*>(gbase + (uint64_t)k * a.RCU +
+                                                  ((uint64_t)j * a.R + i0) * w) = tmp.p;
     }
   } else if (a.cps) {
     const Idx per = a.R * a.C * a.cps;
@@ -343,15 +366,14 @@ __device__ __forceinline__ void col_body(const Args<Idx>& a, const Idx f0, unsig
       const Idx i = a.fvpr.div(rem);
       const Idx jj0 = (rem - i * a.vpr) * VEC;
       const Foot<Idx, OP_COL> f = decode<Idx, OP_COL>(a, f0 + k);
-      Word tmp[VEC];
+      Packed<Word, VEC> tmp;
 #pragma unroll
       for (int v = 0; v < VEC; ++v) {
         const Idx s = col_src<Idx, MAP>(a, i, f.row + jj0 + v);
-        tmp[v] = *reinterpret_cast<const Word*>(lds + k * a.fpl + s * a.rs + (jj0 + v) * w);
+        tmp.e[v] = *reinterpret_cast<const Word*>(lds + k * a.fpl + s * a.rs + (jj0 + v) * w);
       }
-      *reinterpret_cast<Vec<VEC * sizeof(Word)>*>(a.data + f.addr + (uint64_t)i * a.CU +
-                                                  (uint64_t)jj0 * w) =
-          *reinterpret_cast<const Vec<VEC * sizeof(Word)>*>(tmp);
+      *reinterpret_cast<typename Packed<Word, VEC>::P*>(a.data + f.addr + (uint64_t)i * a.CU +
+                                                  (uint64_t)jj0 * w) = tmp.p;
     }
   } else {
     const Idx n = nf * a.st_per;
@@ -430,12 +452,11 @@ __device__ __forceinline__ void row_body(const Args<Idx>& a, const Idx f0, unsig
       const Idx k = a.fst.div(t);
       const Idx j0 = (t - k * a.st_per) * VEC;
       const Idx i = a.fR.mod(f0 + k);
-      Word tmp[VEC];
+      Packed<Word, VEC> tmp;
 #pragma unroll
       for (int v = 0; v < VEC; ++v)
-        tmp[v] = reinterpret_cast<const Word*>(lds)[k * a.C + row_src<Idx, MAP>(a, i, j0 + v)];
-      *reinterpret_cast<Vec<VEC * sizeof(Word)>*>(gbase + (uint64_t)k * a.CU + (uint64_t)j0 * w) =
-          *reinterpret_cast<const Vec<VEC * sizeof(Word)>*>(tmp);
+        tmp.e[v] = reinterpret_cast<const Word*>(lds)[k * a.C + row_src<Idx, MAP>(a, i, j0 + v)];
+      *reinterpret_cast<typename Packed<Word, VEC>::P*>(gbase + (uint64_t)k * a.CU + (uint64_t)j0 * w) = tmp.p;
     }
   } else {
     const Idx n = nf * a.st_per;
@@ -615,13 +636,12 @@ __global__ __launch_bounds__(256) void k_tile_oop(Args<Idx> a, TileGeom tg) {
     const Idx n = nc * vpr;
     for (Idx t = tid; t < n; t += nt) {
       const Idx jr = t / vpr, iv = (t - (t / vpr) * vpr) * VEC;
-      Word tmp[VEC];
+      Packed<Word, VEC> tmp;
 #pragma unroll
       for (int v = 0; v < VEC; ++v)
-        tmp[v] = *reinterpret_cast<const Word*>(lds + (iv + v) * tg.rs + jr * w);
-      *reinterpret_cast<Vec<VEC * sizeof(Word)>*>(
-          a.data + f.addr + ((uint64_t)(j0 + jr) * a.R + i0 + iv) * a.U) =
-          *reinterpret_cast<const Vec<VEC * sizeof(Word)>*>(tmp);
+        tmp.e[v] = *reinterpret_cast<const Word*>(lds + (iv + v) * tg.rs + jr * w);
+      *reinterpret_cast<typename Packed<Word, VEC>::P*>(
+          a.data + f.addr + ((uint64_t)(j0 + jr) * a.R + i0 + iv) * a.U) = tmp.p;
     }
   } else {
     const Idx cw = a.wu;
@@ -1041,14 +1061,14 @@ struct CycArgs {
   uint32_t nseg, nslots, nmaps, chunk_bytes;
   uint32_t lane_log2;  // log2 of the lanes per item (6: one item per wave)
   uint64_t B, items, chunks;
-  CycMapDev maps[4];
+  CycMapDev maps[kMaxCycleMaps];
   FD64 fchunks, fnseg;
 };
 
 // source position of cell position p: the pass's maps composed in gather order
 __device__ __forceinline__ uint32_t cyc_src(const CycArgs& a, uint32_t p) {
 #pragma unroll
-  for (int k = 3; k >= 0; --k) {
+  for (int k = kMaxCycleMaps - 1; k >= 0; --k) {
     if (k >= (int)a.nmaps) continue;
     const CycMapDev& M = a.maps[k];
     const uint32_t b = M.fbc.div(p), q = p - b * M.bc;
@@ -1080,13 +1100,7 @@ typedef uint32_t U32x4 __attribute__((ext_vector_type(4)));
 template <int NT, typename Word>
 __device__ __forceinline__ Word ld_nt(const char* p) {
   if constexpr ((NT & 1) && sizeof(Word) == 16) {
-    const U32x4 v = __builtin_nontemporal_load(reinterpret_cast<const U32x4*>(p));
-    Word w;
-    w.v[0] = v.x;
-    w.v[1] = v.y;
-    w.v[2] = v.z;
-    w.v[3] = v.w;
-    return w;
+    return __builtin_nontemporal_load(reinterpret_cast<const U32x4*>(p));
   } else if constexpr ((NT & 1) && sizeof(Word) < 16) {
     return __builtin_nontemporal_load(reinterpret_cast<const Word*>(p));
   } else {
@@ -1096,12 +1110,7 @@ __device__ __forceinline__ Word ld_nt(const char* p) {
 template <int NT, typename Word>
 __device__ __forceinline__ void st_nt(char* p, const Word& w) {
   if constexpr ((NT & 2) && sizeof(Word) == 16) {
-    U32x4 v;
-    v.x = w.v[0];
-    v.y = w.v[1];
-    v.z = w.v[2];
-    v.w = w.v[3];
-    __builtin_nontemporal_store(v, reinterpret_cast<U32x4*>(p));
+    __builtin_nontemporal_store(w, reinterpret_cast<U32x4*>(p));
   } else if constexpr ((NT & 2) && sizeof(Word) < 16) {
     __builtin_nontemporal_store(w, reinterpret_cast<Word*>(p));
   } else {
@@ -1337,9 +1346,9 @@ hipError_t prepare_cycle_args(const Op& op, char* data, char* scratch, int64_t s
   a.boundary = scratch;
   a.nseg = (uint32_t)t.segs.size();
   a.nslots = (uint32_t)t.slots.size();
-  if (t.maps.empty() || t.maps.size() > 4) return hipErrorInvalidValue;
+  if (t.maps.empty() || t.maps.size() > (size_t)kMaxCycleMaps) return hipErrorInvalidValue;
   a.nmaps = (uint32_t)t.maps.size();
-  for (size_t k = 0; k < 4; ++k) {
+  for (size_t k = 0; k < (size_t)kMaxCycleMaps; ++k) {
     const CycleMap M = k < t.maps.size() ? t.maps[k] : CycleMap{1, 1, 1, 1};
     a.maps[k].bc = (uint32_t)M.bc;
     a.maps[k].R = (uint32_t)M.R;
@@ -1471,25 +1480,24 @@ __device__ __forceinline__ void block_body(const BlkArgs& a, const uint64_t f0, 
     const bool inner = a.ndig && a.frad[0].d % V == 0 && a.n % V == 0;
     for (uint32_t t = tid; t < total; t += nt) {
       const uint32_t o0 = t * V;
-      Word tmp[V];
+      Packed<Word, V> tmp;
       if (inner) {
         // the V cells lie in one block and differ in its fastest output digit
         const uint32_t k = a.fn.div(o0);
         const uint32_t s0 = blk_src(a, o0 - k * a.n);
 #pragma unroll
         for (int v = 0; v < V; ++v)
-          tmp[v] = *reinterpret_cast<const Word*>(
+          tmp.e[v] = *reinterpret_cast<const Word*>(
               lds + k * a.Fl + lds_off((s0 + v * a.stride[0]) * w, a.pad));
       } else {
 #pragma unroll
         for (int v = 0; v < V; ++v) {
           const uint32_t o = o0 + v, k = a.fn.div(o);
-          tmp[v] = *reinterpret_cast<const Word*>(lds + k * a.Fl +
+          tmp.e[v] = *reinterpret_cast<const Word*>(lds + k * a.Fl +
                                                   lds_off(blk_src(a, o - k * a.n) * w, a.pad));
         }
       }
-      *reinterpret_cast<Vec<V * sizeof(Word)>*>(out + (uint64_t)o0 * w) =
-          *reinterpret_cast<const Vec<V * sizeof(Word)>*>(tmp);
+      *reinterpret_cast<typename Packed<Word, V>::P*>(out + (uint64_t)o0 * w) = tmp.p;
     }
   } else {
     const uint32_t per = a.n * a.wpc;
@@ -1687,15 +1695,13 @@ __device__ __forceinline__ void lds_putw(unsigned char* p, const Word& v) {
 }
 template <typename Word>
 __device__ __forceinline__ Word lds_getw(const unsigned char* p) {
-  if constexpr (sizeof(Word) >= 8) {
-    if (((uintptr_t)p & (sizeof(Word) - 1)) == 0) return *reinterpret_cast<const Word*>(p);
-    union {
-      uint32_t u[sizeof(Word) / 4];
-      Word w;
-    } x;
-#pragma unroll
-    for (int i = 0; i < (int)(sizeof(Word) / 4); ++i) x.u[i] = reinterpret_cast<const uint32_t*>(p)[i];
-    return x.w;
+  if (sizeof(Word) < 8 || ((uintptr_t)p & (sizeof(Word) - 1)) == 0)
+    return *reinterpret_cast<const Word*>(p);
+  const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
+  if constexpr (sizeof(Word) == 16) {
+    return W16{q[0], q[1], q[2], q[3]};
+  } else if constexpr (sizeof(Word) == 8) {
+    return (Word)q[0] | ((Word)q[1] << 32);
   } else {
     return *reinterpret_cast<const Word*>(p);
   }
@@ -1705,17 +1711,19 @@ __device__ __forceinline__ uint64_t seg_addr(const SegArgs& a, uint32_t s, uint3
   uint64_t addr = 0;
 #pragma unroll
   for (int d = 0; d < 6; ++d) {
-    if (d >= (int)a.nin) break;
-    const uint32_t q = a.fin[d].div(s);
-    addr += (uint64_t)(s - q * a.fin[d].d) * a.sin[d];
-    s = q;
+    if (d < (int)a.nin) {
+      const uint32_t q = a.fin[d].div(s);
+      addr += (uint64_t)(s - q * a.fin[d].d) * a.sin[d];
+      s = q;
+    }
   }
 #pragma unroll
   for (int d = 0; d < 6; ++d) {
-    if (d >= (int)a.nout) break;
-    const uint32_t q = a.fout[d].div(b);
-    addr += (uint64_t)(b - q * a.fout[d].d) * a.sout[d];
-    b = q;
+    if (d < (int)a.nout) {
+      const uint32_t q = a.fout[d].div(b);
+      addr += (uint64_t)(b - q * a.fout[d].d) * a.sout[d];
+      b = q;
+    }
   }
   return addr;
 }
@@ -1763,13 +1771,10 @@ __device__ __forceinline__ void seg_body(const SegArgs& a, const uint32_t f0, un
     Word val;
     if constexpr (V > 1) {
       using Elem = typename UIntOf<sizeof(Word) / V>::T;
-      union {
-        Elem e[V];
-        Word w;
-      } pk;
+      Packed<Elem, V> pk;
 #pragma unroll
       for (int v = 0; v < V; ++v) pk.e[v] = *reinterpret_cast<const Elem*>(l + (x * V + v) * a.rs);
-      val = pk.w;
+      val = pk.p;
     } else {
       const uint32_t i = a.fwpe.div(x);
       val = lds_getw<Word>(l + i * a.rs + (x - i * a.fwpe.d) * w);
@@ -1973,7 +1978,7 @@ __device__ __forceinline__ void batch_body(const BatchOpDev& op, uint32_t f0, un
                                                 item.f0, lds);                              \
     break;
 
-__global__ __launch_bounds__(512) void k_batch(const BatchOpDev* __restrict__ ops,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_batch(const BatchOpDev* __restrict__ ops,
                                                const BatchItemDev* __restrict__ items,
                                                uint32_t nitems) {
   extern __shared__ __align__(16) unsigned char lds[];
@@ -2022,6 +2027,26 @@ __global__ __launch_bounds__(512) void k_batch(const BatchOpDev* __restrict__ op
     __syncthreads();  // the next item reuses the LDS
   }
 }
+
+// the items without LDS (cycle following and boundary saves) in a kernel of
+// their own: it needs a fraction of k_batch's registers, so more waves of
+// these latency-bound items stay resident
+__global__ __launch_bounds__(512) void k_batch_cycles(const BatchOpDev* __restrict__ ops,
+                                                      const BatchItemDev* __restrict__ items,
+                                                      uint32_t nitems) {
+  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const BatchItemDev item = items[it];
+    const BatchOpDev& op = ops[item.op];
+    switch (op.variant) {
+      EH_CYC(0)
+      EH_CYC(1)
+      EH_CYC(2)
+      EH_CYC(3)
+      EH_CYC(4)
+      default: break;
+    }
+  }
+}
 #undef EH_BLK
 #undef EH_SEG
 #undef EH_CYC
@@ -2060,6 +2085,7 @@ struct BatchLaunch {
   struct Part {
     uint32_t offset, count, grid;
     int64_t lds;
+    bool cycles;  // only cycle and boundary-save items: k_batch_cycles
   };
   BatchOpDev* d_ops = nullptr;
   BatchItemDev* d_items = nullptr;
@@ -2207,17 +2233,18 @@ int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, cons
   BatchLaunch* L = new BatchLaunch;
   L->threads = threads;
   std::vector<BatchItemDev> hitems;
-  auto add_part = [&](const std::vector<BatchItemDev>& v, int64_t plds, int64_t per_cu) {
+  auto add_part = [&](const std::vector<BatchItemDev>& v, int64_t plds, int64_t per_cu,
+                      bool cyc) {
     if (v.empty()) return;
     const uint32_t n = (uint32_t)v.size();
     const uint32_t grid =
         grid_per_cu > 0 ? (uint32_t)std::min<int64_t>(n, (int64_t)cus * per_cu) : n;
-    L->parts.push_back({(uint32_t)hitems.size(), n, grid, plds});
+    L->parts.push_back({(uint32_t)hitems.size(), n, grid, plds, cyc});
     hitems.insert(hitems.end(), v.begin(), v.end());
   };
   // without LDS, up to 2048 threads per CU stay resident
-  add_part(saves, 0, 2048 / threads);
-  add_part(cycles, 0, 2048 / threads);
+  add_part(saves, 0, 2048 / threads, true);
+  add_part(cycles, 0, 2048 / threads, true);
   // LDS items (sorted by footprint, largest first) in classes of at most
   // 40 KiB, 80 KiB and the rest
   for (int64_t cap : {(int64_t)1 << 40, (int64_t)80 * 1024, (int64_t)40 * 1024}) {
@@ -2230,7 +2257,7 @@ int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, cons
         clds = std::max(clds, it.first);
       }
     }
-    add_part(cls, clds, grid_per_cu);
+    add_part(cls, clds, grid_per_cu, false);
   }
   if (hipMalloc((void**)&L->d_ops, hops.size() * sizeof(BatchOpDev)) != hipSuccess ||
       hipMalloc((void**)&L->d_items, hitems.size() * sizeof(BatchItemDev)) != hipSuccess) {
@@ -2252,9 +2279,14 @@ hipError_t batch_launch_enqueue(const BatchLaunch* L, hipStream_t st) {
   if (!L) return hipSuccess;
   // the boundary-save part comes first: every cycle pass's cells are saved
   // before any of them writes
-  for (const BatchLaunch::Part& p : L->parts)
-    hipLaunchKernelGGL(k_batch, dim3(p.grid ? p.grid : 1), dim3(L->threads), (size_t)p.lds, st,
-                       L->d_ops, L->d_items + p.offset, p.count);
+  for (const BatchLaunch::Part& p : L->parts) {
+    if (p.cycles)
+      hipLaunchKernelGGL(k_batch_cycles, dim3(p.grid ? p.grid : 1), dim3(L->threads), 0, st,
+                         L->d_ops, L->d_items + p.offset, p.count);
+    else
+      hipLaunchKernelGGL(k_batch, dim3(p.grid ? p.grid : 1), dim3(L->threads), (size_t)p.lds, st,
+                         L->d_ops, L->d_items + p.offset, p.count);
+  }
   return hipGetLastError();
 }
 

@@ -531,10 +531,13 @@ constexpr double kCostBlock = 1.1;   // fused block permutation
 // Cells whose size is not a multiple of 16 bytes move in narrower words
 // (0.8-0.9 TB/s measured for 2- and 4-byte words).
 thread_local bool g_cycle_word_cost = true;  // off: every cell width costs the same
+thread_local bool g_cycle_cell_pref = false;  // on: larger cells cost less
 inline double cycle_cost(int64_t U) {
   const int64_t w = pow2_factor(U, 16);
   if (!g_cycle_word_cost) return kCostCycle;
-  return w >= 16 ? kCostCycle : w == 8 ? 2.0 : w == 4 ? 4.0 : 6.0;
+  if (w < 16) return w == 8 ? 2.0 : w == 4 ? 4.0 : 6.0;
+  if (!g_cycle_cell_pref) return kCostCycle;
+  return U >= 512 ? kCostCycle - 0.1 : U >= 256 ? kCostCycle - 0.05 : kCostCycle;
 }
 
 // Widest word (at most 16 bytes) a segmented tile moves: it divides the
@@ -779,6 +782,7 @@ static int64_t* tuning_field(Tuning& t, const char* key) {
   if (!strcmp(key, "reorder")) return &t.reorder;
   if (!strcmp(key, "regroup")) return &t.regroup;
   if (!strcmp(key, "cycle_lmax")) return &t.cycle_lmax;
+  if (!strcmp(key, "cycle_cell_pref")) return &t.cycle_cell_pref;
   return nullptr;
 }
 
@@ -820,6 +824,7 @@ void tuning_apply(const Tuning& t, DeviceProfile& prof) {
   prof.reorder = (int)t.reorder;
   prof.regroup = t.regroup;
   prof.cycle_lmax = t.cycle_lmax;
+  prof.cycle_cell_pref = t.cycle_cell_pref;
 }
 
 void lower_transpose2d(int64_t B, int64_t R, int64_t C, int64_t U, uint64_t base, int stage,
@@ -1297,7 +1302,7 @@ bool regroup2_plan(const Regroup& g, int i, int j, bool z_out_order, bool tile_l
   }
   std::vector<Stage> s1, s3;
   if (!fewest_stages(sp.c1, sp.t1, s1) || !fewest_stages(sp.c3, sp.t3, s3)) return false;
-  if (s1.size() + s3.size() > 4) return false;
+  if (s1.size() + s3.size() > (size_t)kMaxCycleMaps) return false;
   std::vector<CycleMap> maps;
   int64_t N = 1;
   for (int side = 0; side < 2; ++side)
@@ -1542,6 +1547,7 @@ static void finalize_plan(Plan& plan, double alpha, int64_t reserve, const Devic
 int make_plan(const int64_t* dims_in, const int* perm1, int rank, size_t elem, int n_sub,
               double alpha, const DeviceProfile& prof, int reorder, Plan& plan) {
   g_cycle_word_cost = prof.cycle_word_cost != 0;
+  g_cycle_cell_pref = prof.cycle_cell_pref != 0;
   // Search: decomposer order (pre/post, or the one asked for) x how hard
   // staged passes are avoided; cycle passes that later compose with their
   // neighbours make a split lowering cheaper than a staged one, which only

@@ -67,6 +67,7 @@ struct CycleSeg {
 struct CycleMap {
   int64_t bc, R, C, m;
 };
+constexpr int kMaxCycleMaps = 5;  // maps one cycle pass composes
 struct CycleTable {
   int64_t lmax = 0;
   // maps applied in pass order; the pass's source of position p is
@@ -160,6 +161,7 @@ struct DeviceProfile {
   int reorder = 0;                // 0 cheapest of pre/post (mix breaks ties), 1 pre, 2 post, 3 mix
   int64_t regroup = 1;            // with reorder 0: also plan cell pass / tile / cell pass
   int64_t cycle_lmax = 0;         // > 0: cycle segments of at most this many positions
+  int64_t cycle_cell_pref = 0;    // 1: cycle passes of larger cells cost less
   double staged_mult = 1.0;       // planner-internal: weight of staged passes in the search
 };
 
@@ -181,6 +183,7 @@ struct Tuning {
   int64_t reorder = 0;
   int64_t regroup = 1;
   int64_t cycle_lmax = 0;
+  int64_t cycle_cell_pref = 0;
   uint64_t epoch = 0;  // bumped on every change (plan caches key on it)
 };
 Tuning tuning_get();
