@@ -176,6 +176,8 @@ int eithot_plan_segments(const int64_t* dims, const int* perm, int order, size_t
  *                    for cells of at least 64 words)
  *   "cycle_lmax"     > 0: cycle segments of at most this many positions
  *                    (0: chosen per pass)
+ *   "cycle_cell_pref" cycle passes of cells of 256 B and more cost less in
+ *                    the planner (0 default: not faster on MI355X so far)
  *   "cycle_word_cost" cycle passes over cells that are not whole 16-byte
  *                    words cost more in the planner (1) or the same (0)
  *   "threads", "threads_big"  workgroup sizes (256, 512; multiples of 64, at most 512)
