@@ -1593,8 +1593,10 @@ hipError_t prepare_block_args(const Op& op, char* data, const LaunchCfg& cfg, Bl
   int64_t K = pick_K(op.B, a.Fl, cfg.lds_target);
   while (K > 1 && K * a.Fl > 160 * 1024) --K;
   // a multiple of the blocks that make 16-byte runs and whole 16-byte stores
+  // (past the LDS target when that many blocks fit the workgroup's LDS)
   int64_t mK = 16 / gcd64(F, 16);
   if (vec) mK = mK / gcd64(mK, V / gcd64(op.R, V)) * (V / gcd64(op.R, V));
+  if (K < mK && mK <= op.B && mK * (int64_t)a.Fl <= 160 * 1024) K = mK;
   if (K >= mK) K -= K % mK;
   a.K = (uint32_t)K;
   a.fnw.init(a.n * a.wpc);
