@@ -8,6 +8,7 @@
 #include <map>
 #include <mutex>
 #include <sstream>
+#include <thread>
 #include <tuple>
 
 #include "eithot_status.h"
@@ -232,6 +233,111 @@ struct MapDiv {
 
 std::mutex g_cyc_mu;
 std::map<std::vector<int64_t>, std::shared_ptr<CycleTable>> g_cyc_cache;  // recent tables
+
+// Tables over at least this many cells are walked on several threads.
+constexpr uint64_t kParallelWalkCells = 1ull << 20;
+
+// The cycles that hold a hashed boundary position (about one position in
+// 64) are walked on several threads, piece by piece: a piece starts at a
+// boundary and follows p -> src(p) up to the next boundary.  Every position
+// of such a cycle lies on exactly one piece.  One thread then chains the
+// pieces of each cycle and cuts the cycle into segments of lmax positions,
+// exactly as a single walk would (a segment start inside a piece is found
+// by stepping from the piece's start).  Cycles without a boundary stay
+// unmarked in `seen` for the sequential walk.
+template <typename Src>
+void walk_parallel(const Src& src, uint64_t N, int64_t lmax, std::vector<uint64_t>& seen,
+                   CycleTable& t) {
+  constexpr uint64_t kPiece = 64;
+  auto boundary = [&](uint64_t p) { return ((p * 0x9E3779B97F4A7C15ull) >> 40) % kPiece == 0; };
+  struct Piece {
+    uint32_t start, len, next;  // next: the start of the following piece
+  };
+  const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::vector<Piece>> out(nt);
+  auto work = [&](unsigned w) {
+    const uint64_t lo = N * w / nt, hi = N * (w + 1) / nt;
+    std::vector<Piece>& v = out[w];
+    for (uint64_t p = lo; p < hi; ++p) {
+      if (!boundary(p)) continue;
+      uint64_t q = p, k = 0;
+      do {
+        __atomic_fetch_or(&seen[q >> 6], 1ull << (q & 63), __ATOMIC_RELAXED);
+        q = src(q);
+        ++k;
+      } while (q != p && !boundary(q));
+      v.push_back({(uint32_t)p, (uint32_t)k, (uint32_t)q});
+    }
+  };
+  std::vector<std::thread> th;
+  for (unsigned w = 1; w < nt; ++w) th.emplace_back(work, w);
+  work(0);
+  for (std::thread& x : th) x.join();
+  std::vector<Piece> all;
+  for (const auto& v : out) all.insert(all.end(), v.begin(), v.end());
+  std::sort(all.begin(), all.end(), [](const Piece& a, const Piece& b) { return a.start < b.start; });
+  // index of each piece's successor, on the threads
+  std::vector<uint32_t> nxt(all.size());
+  auto link = [&](unsigned w) {
+    const size_t lo = all.size() * w / nt, hi = all.size() * (w + 1) / nt;
+    for (size_t i = lo; i < hi; ++i)
+      nxt[i] = (uint32_t)(std::lower_bound(all.begin(), all.end(), all[i].next,
+                                           [](const Piece& a, uint32_t x) { return a.start < x; }) -
+                          all.begin());
+  };
+  th.clear();
+  for (unsigned w = 1; w < nt; ++w) th.emplace_back(link, w);
+  link(0);
+  for (std::thread& x : th) x.join();
+  const uint64_t L = (uint64_t)lmax;
+  std::vector<uint8_t> used(all.size(), 0);
+  std::vector<uint32_t> chain;
+  std::vector<uint64_t> at_off;  // cycle position of each chained piece's start
+  std::vector<uint32_t> starts;
+  for (size_t i0 = 0; i0 < all.size(); ++i0) {
+    if (used[i0]) continue;
+    chain.clear();
+    at_off.clear();
+    uint64_t total = 0;
+    for (uint32_t i = (uint32_t)i0; !used[i]; i = nxt[i]) {
+      used[i] = 1;
+      chain.push_back(i);
+      at_off.push_back(total);
+      total += all[i].len;
+    }
+    if (total == 1) continue;  // a fixed point
+    if (total <= L) {
+      t.segs.push_back({all[i0].start, (uint32_t)total, -1, 0});
+      continue;
+    }
+    // segment starts every lmax positions along the cycle from all[i0].start,
+    // each stepped to from the start of the piece that holds it (threads)
+    const uint64_t n64 = (total + L - 1) / L;
+    starts.assign(n64, 0);
+    auto cut = [&](unsigned w, unsigned parts) {
+      const uint64_t lo = n64 * w / parts, hi = n64 * (w + 1) / parts;
+      for (uint64_t c = lo; c < hi; ++c) {
+        const uint64_t at = c * L;
+        const size_t k = (size_t)(std::upper_bound(at_off.begin(), at_off.end(), at) -
+                                  at_off.begin()) - 1;
+        uint64_t q = all[chain[k]].start;
+        for (uint64_t x = at_off[k]; x < at; ++x) q = src(q);
+        starts[c] = (uint32_t)q;
+      }
+    };
+    const unsigned parts = n64 >= 4096 ? nt : 1;
+    th.clear();
+    for (unsigned w = 1; w < parts; ++w) th.emplace_back(cut, w, parts);
+    cut(0, parts);
+    for (std::thread& x : th) x.join();
+    const uint32_t n = (uint32_t)starts.size(), first_slot = (uint32_t)t.slots.size();
+    for (uint32_t k = 0; k < n; ++k) {
+      const uint64_t len = k + 1 < n ? L : total - (uint64_t)(n - 1) * L;
+      t.slots.push_back(starts[k]);
+      t.segs.push_back({starts[k], (uint32_t)len, (int32_t)(first_slot + (k + 1) % n), 0});
+    }
+  }
+}
 }  // namespace
 
 std::shared_ptr<CycleTable> build_cycle_table_maps(const std::vector<CycleMap>& maps, int64_t N,
@@ -261,10 +367,11 @@ std::shared_ptr<CycleTable> build_cycle_table_maps(const std::vector<CycleMap>& 
     }
     return p;
   };
-  // one walk per cycle over a visited bitset; a cycle longer than lmax is
-  // cut into segments of lmax positions (the last one shorter), each
-  // starting at a recorded position with a boundary slot
   std::vector<uint64_t> seen(((size_t)N + 63) / 64, 0);
+  if ((uint64_t)N >= kParallelWalkCells) walk_parallel(src, (uint64_t)N, lmax, seen, *t);
+  // one walk per remaining cycle over the visited bitset; a cycle longer
+  // than lmax is cut into segments of lmax positions (the last one
+  // shorter), each starting at a recorded position with a boundary slot
   std::vector<uint32_t> starts;
   for (uint64_t p0 = 0; p0 < (uint64_t)N; ++p0) {
     if (seen[p0 >> 6] >> (p0 & 63) & 1) continue;

@@ -34,7 +34,9 @@ def ch():
 
 
 SHAPES = [(2, 3), (3, 2), (8, 36), (36, 8), (48, 320), (64, 100), (7, 11), (16, 16), (1, 9),
-          (1440, 8), (11520, 64)]
+          (1440, 8), (11520, 64),
+          # at least 2^20 cells: walked on several threads from hashed boundaries
+          (1024, 1536), (4097, 513)]
 
 
 @pytest.mark.parametrize("R,C", SHAPES)
@@ -46,7 +48,8 @@ def test_cycle_tables_cover_every_moved_position_once(ch, R, C, lmax):
 @pytest.mark.parametrize("mode", [0, 1, 2, 3])
 @pytest.mark.parametrize("B,R,C,U,lmax,chunk", [
     (3, 8, 36, 48, 1 << 30, 16), (2, 36, 8, 40, 7, 16), (1, 48, 320, 32, 64, 16),
-    (2, 13, 17, 24, 3, 8), (1, 64, 100, 16, 9, 16), (4, 5, 7, 64, 1, 64), (1, 120, 64, 16, 100, 16)])
+    (2, 13, 17, 24, 3, 8), (1, 64, 100, 16, 9, 16), (4, 5, 7, 64, 1, 64), (1, 120, 64, 16, 100, 16),
+    (1, 1024, 1100, 4, 64, 4), (1, 2049, 1023, 2, 256, 2)])
 def test_cycle_emulation_equals_transpose(ch, mode, B, R, C, U, lmax, chunk):
     vol = B * R * C * U
     a = O.fill(vol, 1, 17 + mode, 1)
@@ -64,7 +67,8 @@ def test_cycle_emulation_equals_transpose(ch, mode, B, R, C, U, lmax, chunk):
     ([(96, 8, 12, 1), (96, 4, 3, 8)], 16, 7),             # two transposes of one 96-cell block
     ([(30, 5, 6, 1), (60, 2, 3, 10), (120, 4, 30, 1)], 8, 1 << 30),
     ([(64, 64, 1, 1), (64, 8, 8, 1)], 32, 5),             # identity map, then a square one
-    ([(240, 16, 15, 1), (240, 15, 16, 1)], 16, 11)])      # a transpose and its inverse
+    ([(240, 16, 15, 1), (240, 15, 16, 1)], 16, 11),       # a transpose and its inverse
+    ([(1048576, 1024, 1024, 1), (1048576, 32, 2048, 16)], 4, 256)])  # 2^21 cells, threaded walk
 def test_composite_cycle_maps_equal_sequential_transposes(ch, mode, maps, U, lmax):
     N = max(bc for bc, _, _, _ in maps) * 2
     vol = N * U
