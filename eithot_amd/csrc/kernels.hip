@@ -1792,6 +1792,16 @@ __global__ __launch_bounds__(512) void k_segtile(SegArgs a, int remap) {
   seg_body<Word, V>(a, xcd_logical(blockIdx.x, gridDim.x, remap) * a.K, lds);
 }
 
+// Workgroup size of a segmented tile pass: enough threads that each issues
+// one full round of UNR loads for the workgroup's words (whole waves, at
+// most 512), unless the segtile_threads knob fixes it.
+int segtile_threads(const SegArgs& a, const LaunchCfg& cfg) {
+  if (cfg.segtile_threads > 0) return cfg.segtile_threads;
+  const uint64_t words = (uint64_t)a.K * a.wpm;
+  const uint64_t t = (words + UNR - 1) / UNR;
+  return (int)std::min<uint64_t>(512, std::max<uint64_t>(64, (t + 63) / 64 * 64));
+}
+
 // Word size of a segmented tile pass: the widest (at most 16 bytes) that
 // divides the segments, both row lengths and the data pointer, and that
 // divides the unit or is a multiple of it.
@@ -1878,7 +1888,7 @@ hipError_t prepare_seg_args(const Op& op, char* data, const LaunchCfg& cfg, SegA
 template <typename Word, int V>
 hipError_t launch_seg_wv(const SegArgs& a, int64_t lds, int64_t groups, const LaunchCfg& cfg,
                          hipStream_t stream) {
-  const int threads = lds > 96 * 1024 ? cfg.threads_big : cfg.threads;
+  const int threads = segtile_threads(a, cfg);
   hipLaunchKernelGGL((k_segtile<Word, V>), dim3((uint32_t)groups), dim3(threads), (size_t)lds,
                      stream, a, cfg.xcd_remap);
   return hipGetLastError();

@@ -17,6 +17,7 @@ struct LaunchCfg {
   int threads_big = 512;           // workgroups whose LDS footprint exceeds 96 KiB
   int xcd_remap = 1;               // keep consecutive footprints on one XCD
   int cycle_variant = 0;           // cycle-following kernel shape (0..3)
+  int segtile_threads = 0;         // > 0: workgroup size of segmented tile passes
 };
 
 // One pass of `op` over the tensor at `data`, on `stream`.  Staged ops use

@@ -890,6 +890,7 @@ static int64_t* tuning_field(Tuning& t, const char* key) {
   if (!strcmp(key, "regroup")) return &t.regroup;
   if (!strcmp(key, "cycle_lmax")) return &t.cycle_lmax;
   if (!strcmp(key, "cycle_cell_pref")) return &t.cycle_cell_pref;
+  if (!strcmp(key, "segtile_threads")) return &t.segtile_threads;
   return nullptr;
 }
 
@@ -898,7 +899,8 @@ int tuning_set(const char* key, int64_t value) {
   Tuning t = g_tune;
   int64_t* f = tuning_field(t, key);
   if (!f || value < 0) return EITHOT_ERR_INVALID;
-  const bool threads = f == &t.threads || f == &t.threads_big;
+  const bool threads = f == &t.threads || f == &t.threads_big ||
+                       (f == &t.segtile_threads && value != 0);
   if (threads && (value < 64 || value > 512 || value % 64)) return EITHOT_ERR_INVALID;
   if (f == &t.reorder && value > 3) return EITHOT_ERR_INVALID;
   if (f == &t.cycle_variant && value > 7) return EITHOT_ERR_INVALID;

@@ -184,6 +184,7 @@ struct Tuning {
   int64_t regroup = 1;
   int64_t cycle_lmax = 0;
   int64_t cycle_cell_pref = 0;
+  int64_t segtile_threads = 0;
   uint64_t epoch = 0;  // bumped on every change (plan caches key on it)
 };
 Tuning tuning_get();

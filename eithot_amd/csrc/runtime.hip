@@ -90,6 +90,7 @@ LaunchCfg Device::launch_cfg() const {
   c.threads_big = (int)t.threads_big;
   c.xcd_remap = (int)t.xcd_remap;
   c.cycle_variant = (int)t.cycle_variant;
+  c.segtile_threads = (int)t.segtile_threads;
   return c;
 }
 

@@ -181,6 +181,8 @@ int eithot_plan_segments(const int64_t* dims, const int* perm, int order, size_t
  *   "cycle_word_cost" cycle passes over cells that are not whole 16-byte
  *                    words cost more in the planner (1) or the same (0)
  *   "threads", "threads_big"  workgroup sizes (256, 512; multiples of 64, at most 512)
+ *   "segtile_threads" workgroup size of segmented tile passes (0: one round of
+ *                    8 loads per thread, at most 512; else a multiple of 64)
  *   "xcd_remap"      keep consecutive footprints on one XCD (1)
  * Returns EITHOT_ERR_INVALID for an unknown key or an out-of-range value. */
 int eithot_set_tuning(const char* key, int64_t value);
