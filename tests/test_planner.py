@@ -183,6 +183,24 @@ def test_regroup_plans_execute_correctly():
     assert differ > 10
 
 
+@pytest.mark.parametrize("dims,perm1,elem", [
+    ([1, 1, 1], [1, 2, 3], 4), ([1, 5, 1, 7], [4, 3, 2, 1], 4), ([3, 1, 4], [3, 2, 1], 2),
+    ([2, 3, 4, 5], [1, 2, 4, 3], 8), ([2] * 12, list(range(12, 0, -1)), 4),
+    ([3, 2] * 6, list(range(12, 0, -1)), 1), ([64, 2, 3], [1, 3, 2], 4),
+    ([8, 1, 1, 16], [4, 2, 3, 1], 2), ([5, 7], [1, 2], 4)])
+def test_plan_edge_shapes(dims, perm1, elem):
+    # unit dims, identity, a leading dim that stays, rank 12, with and without
+    # the sub-tensor partition, at tight and loose budgets
+    vol = int(np.prod(dims))
+    for alpha in (0.01, 0.05, 1.0):
+        for nsub in (1, 4):
+            ops, _ = E.plan_describe(dims, perm1, elem, nsub, alpha)
+            a = O.fill(vol, elem, 2, 1)
+            b = a.copy()
+            simulate(b, ops)
+            assert np.array_equal(b, O.permute(a, dims, [p - 1 for p in perm1], elem)), (alpha, nsub)
+
+
 COMPOSITE_CASES = [([3, 14, 2, 15, 58, 20], [5, 2, 6, 1, 3, 4], 4),
                    ([12, 2, 25, 14, 53, 3], [3, 6, 4, 1, 5, 2], 8),
                    ([12, 9, 14, 28, 53], [2, 1, 5, 4, 3], 8),
