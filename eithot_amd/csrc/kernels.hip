@@ -849,7 +849,14 @@ struct LdsLauncher {
   hipStream_t st;
   template <typename Word, int KM, int V>
   hipError_t run() {
-    const dim3 grid((uint32_t)p.groups), block(p.threads);
+    // standalone launches: enough threads for one round of UNR 16-byte loads
+    // of the workgroup's footprints when the knob asks for it
+    int threads = p.threads;
+    if (cfg.lds_threads_auto) {
+      const int64_t words = (p.lds + 15) / 16;
+      threads = (int)std::min<int64_t>(512, std::max<int64_t>(64, ((words + UNR - 1) / UNR + 63) / 64 * 64));
+    }
+    const dim3 grid((uint32_t)p.groups), block(threads);
     if (KM == 0)
       hipLaunchKernelGGL((k_tile<Word, Idx, V>), grid, block, (size_t)p.lds, st, a, cfg.xcd_remap);
     else if (KM <= 4)

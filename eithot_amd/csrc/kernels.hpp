@@ -18,6 +18,7 @@ struct LaunchCfg {
   int xcd_remap = 1;               // keep consecutive footprints on one XCD
   int cycle_variant = 0;           // cycle-following kernel shape (0..3)
   int segtile_threads = 0;         // > 0: workgroup size of segmented tile passes
+  int lds_threads_auto = 1;        // 1: tile/column/row workgroups sized for one load round
 };
 
 // One pass of `op` over the tensor at `data`, on `stream`.  Staged ops use

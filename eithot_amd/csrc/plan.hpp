@@ -185,6 +185,7 @@ struct Tuning {
   int64_t cycle_lmax = 0;
   int64_t cycle_cell_pref = 0;
   int64_t segtile_threads = 0;
+  int64_t lds_threads_auto = 1;
   uint64_t epoch = 0;  // bumped on every change (plan caches key on it)
 };
 Tuning tuning_get();

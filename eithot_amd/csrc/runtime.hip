@@ -91,6 +91,7 @@ LaunchCfg Device::launch_cfg() const {
   c.xcd_remap = (int)t.xcd_remap;
   c.cycle_variant = (int)t.cycle_variant;
   c.segtile_threads = (int)t.segtile_threads;
+  c.lds_threads_auto = (int)t.lds_threads_auto;
   return c;
 }
 
