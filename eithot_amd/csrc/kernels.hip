@@ -1552,7 +1552,11 @@ template <typename Word>
 hipError_t launch_block_w(BlkArgs a, int vec, int64_t lds, int64_t groups, const LaunchCfg& cfg,
                           hipStream_t stream) {
   constexpr int V = 16 / sizeof(Word) > 1 ? 16 / sizeof(Word) : 1;
-  const int threads = lds > 96 * 1024 ? cfg.threads_big : cfg.threads;
+  int threads = lds > 96 * 1024 ? cfg.threads_big : cfg.threads;
+  if (cfg.lds_threads_auto) {
+    const int64_t words = (lds + 15) / 16;
+    threads = (int)std::min<int64_t>(512, std::max<int64_t>(64, ((words + UNR - 1) / UNR + 63) / 64 * 64));
+  }
   if (vec && V > 1)
     hipLaunchKernelGGL((k_block<Word, V>), dim3((uint32_t)groups), dim3(threads), (size_t)lds, stream,
                        a, cfg.xcd_remap);

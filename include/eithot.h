@@ -183,7 +183,7 @@ int eithot_plan_segments(const int64_t* dims, const int* perm, int order, size_t
  *   "threads", "threads_big"  workgroup sizes (256, 512; multiples of 64, at most 512)
  *   "segtile_threads" workgroup size of segmented tile passes (0: one round of
  *                    8 loads per thread, at most 512; else a multiple of 64)
- *   "lds_threads_auto" tile, column and row workgroups sized for one round of
+ *   "lds_threads_auto" tile, column, row and block workgroups sized for one round of
  *                    8 loads per thread (1) or from "threads"/"threads_big" (0)
  *   "xcd_remap"      keep consecutive footprints on one XCD (1)
  * Returns EITHOT_ERR_INVALID for an unknown key or an out-of-range value. */
