@@ -900,6 +900,9 @@ hipError_t launch_staged(Args<Idx> a, const Op& op, char* scratch, int64_t scrat
       }
       while (trl > 0 && ((int64_t)1 << (trl - 1)) >= op.R) --trl;
       while (tcl > 0 && ((int64_t)1 << (tcl - 1)) >= op.C) --tcl;
+      // a side cut to a short matrix gives its share back to the other one
+      while (((int64_t)1 << (trl + tcl + 1)) * op.us <= 16384 && ((int64_t)1 << tcl) < op.C) ++tcl;
+      while (((int64_t)1 << (trl + tcl + 1)) * op.us <= 16384 && ((int64_t)1 << trl) < op.R) ++trl;
       TileGeom tg;
       tg.tr_log2 = trl;
       tg.tc_log2 = tcl;

@@ -37,9 +37,11 @@ def test_catanzaro_lds_shapes(torch_cuda):
 
 def test_staged_passes(torch_cuda):
     # rows longer than LDS force scratch-staged passes (alpha allows them)
-    for dims, perm in (([3, 100003], [2, 1]), ([100003, 3], [2, 1]), ([7, 65537], [2, 1]),
-                       ([2, 3001, 1801, 2], [1, 3, 2, 4])):
-        got, want, peak = run_case(torch_cuda, dims, perm, 4, seed=4, alpha=1.0)
+    for dims, perm, elem in (([3, 100003], [2, 1], 4), ([100003, 3], [2, 1], 4),
+                             ([7, 65537], [2, 1], 4), ([2, 3001, 1801, 2], [1, 3, 2, 4], 4),
+                             # 18-byte units in matrices of 2 rows, then of 2 columns
+                             ([18, 300001, 2], [1, 3, 2], 1), ([18, 2, 300001], [1, 3, 2], 1)):
+        got, want, peak = run_case(torch_cuda, dims, perm, elem, seed=4, alpha=1.0)
         assert_bitexact(got, want, f"{dims}")
         assert peak <= 1.0 * got.nbytes
 
