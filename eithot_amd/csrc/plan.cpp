@@ -237,6 +237,24 @@ std::map<std::vector<int64_t>, std::shared_ptr<CycleTable>> g_cyc_cache;  // rec
 // Tables over at least this many cells are walked on several threads.
 constexpr uint64_t kParallelWalkCells = 1ull << 20;
 
+// fn(w) for w in [0, n) on up to n threads; a share whose thread cannot be
+// started runs on the calling thread.
+template <typename Fn>
+void run_shares(unsigned n, const Fn& fn) {
+  std::vector<std::thread> th;
+  std::vector<unsigned> inline_shares;
+  for (unsigned w = 1; w < n; ++w) {
+    try {
+      th.emplace_back(fn, w);
+    } catch (...) {
+      inline_shares.push_back(w);
+    }
+  }
+  fn(0u);
+  for (unsigned w : inline_shares) fn(w);
+  for (std::thread& x : th) x.join();
+}
+
 // The cycles that hold a hashed boundary position (about one position in
 // 64) are walked on several threads, piece by piece: a piece starts at a
 // boundary and follows p -> src(p) up to the next boundary.  Every position
@@ -269,10 +287,7 @@ void walk_parallel(const Src& src, uint64_t N, int64_t lmax, std::vector<uint64_
       v.push_back({(uint32_t)p, (uint32_t)k, (uint32_t)q});
     }
   };
-  std::vector<std::thread> th;
-  for (unsigned w = 1; w < nt; ++w) th.emplace_back(work, w);
-  work(0);
-  for (std::thread& x : th) x.join();
+  run_shares(nt, work);
   std::vector<Piece> all;
   for (const auto& v : out) all.insert(all.end(), v.begin(), v.end());
   std::sort(all.begin(), all.end(), [](const Piece& a, const Piece& b) { return a.start < b.start; });
@@ -285,10 +300,7 @@ void walk_parallel(const Src& src, uint64_t N, int64_t lmax, std::vector<uint64_
                                            [](const Piece& a, uint32_t x) { return a.start < x; }) -
                           all.begin());
   };
-  th.clear();
-  for (unsigned w = 1; w < nt; ++w) th.emplace_back(link, w);
-  link(0);
-  for (std::thread& x : th) x.join();
+  run_shares(nt, link);
   const uint64_t L = (uint64_t)lmax;
   std::vector<uint8_t> used(all.size(), 0);
   std::vector<uint32_t> chain;
@@ -326,10 +338,7 @@ void walk_parallel(const Src& src, uint64_t N, int64_t lmax, std::vector<uint64_
       }
     };
     const unsigned parts = n64 >= 4096 ? nt : 1;
-    th.clear();
-    for (unsigned w = 1; w < parts; ++w) th.emplace_back(cut, w, parts);
-    cut(0, parts);
-    for (std::thread& x : th) x.join();
+    run_shares(parts, [&](unsigned w) { cut(w, parts); });
     const uint32_t n = (uint32_t)starts.size(), first_slot = (uint32_t)t.slots.size();
     for (uint32_t k = 0; k < n; ++k) {
       const uint64_t len = k + 1 < n ? L : total - (uint64_t)(n - 1) * L;
