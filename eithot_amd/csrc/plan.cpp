@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <new>
 #include <sstream>
 #include <thread>
 #include <tuple>
@@ -1663,8 +1664,28 @@ static void finalize_plan(Plan& plan, double alpha, int64_t reserve, const Devic
   }
 }
 
+static int make_plan_search(const int64_t* dims_in, const int* perm1, int rank, size_t elem,
+                            int n_sub, double alpha, const DeviceProfile& prof, int reorder,
+                            Plan& plan);
+
+// Planning allocates host tables that can be large (a cycle table spans up
+// to 2^28 cells); running out of host memory is reported, not thrown.
 int make_plan(const int64_t* dims_in, const int* perm1, int rank, size_t elem, int n_sub,
               double alpha, const DeviceProfile& prof, int reorder, Plan& plan) {
+  try {
+    return make_plan_search(dims_in, perm1, rank, elem, n_sub, alpha, prof, reorder, plan);
+  } catch (const std::bad_alloc&) {
+    plan = Plan();
+    return EITHOT_ERR_NOMEM;
+  } catch (...) {
+    plan = Plan();
+    return EITHOT_ERR_INVALID;
+  }
+}
+
+static int make_plan_search(const int64_t* dims_in, const int* perm1, int rank, size_t elem,
+                            int n_sub, double alpha, const DeviceProfile& prof, int reorder,
+                            Plan& plan) {
   g_cycle_word_cost = prof.cycle_word_cost != 0;
   g_cycle_cell_pref = prof.cycle_cell_pref != 0;
   // Search: decomposer order (pre/post, or the one asked for) x how hard
