@@ -20,8 +20,12 @@ def ch():
     deps = [src] + [os.path.join(HERE, "..", "eithot_amd", "csrc", f) for f in ("plan.cpp", "plan.hpp")]
     if not os.path.exists(SO) or os.path.getmtime(SO) < max(os.path.getmtime(d) for d in deps):
         os.makedirs(os.path.dirname(SO), exist_ok=True)
-        subprocess.check_call(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-o", SO, src,
-                               deps[1]])
+        # parallel test workers may build at once: each writes its own file
+        # and renames it into place atomically
+        tmp = f"{SO}.{os.getpid()}.tmp"
+        subprocess.check_call(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-pthread", "-o", tmp,
+                               src, deps[1]])
+        os.replace(tmp, SO)
     L = ctypes.CDLL(SO)
     L.ch_run.restype = ctypes.c_int64
     L.ch_run.argtypes = [ctypes.c_int64] * 6 + [ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p,

@@ -22,8 +22,12 @@ def mh():
     if not os.path.exists(SO) or os.path.getmtime(SO) < max(os.path.getmtime(src),
                                                              os.path.getmtime(hdr)):
         os.makedirs(os.path.dirname(SO), exist_ok=True)
+        # parallel test workers may build at once: each writes its own file
+        # and renames it into place atomically
+        tmp = f"{SO}.{os.getpid()}.tmp"
         subprocess.check_call(["/opt/rocm/bin/hipcc", "-x", "hip", "--offload-arch=gfx950", "-O1",
-                               "-std=c++17", "-fPIC", "-shared", "-o", SO, src])
+                               "-std=c++17", "-fPIC", "-shared", "-o", tmp, src])
+        os.replace(tmp, SO)
     L = ctypes.CDLL(SO)
     L.mh_eval.restype = ctypes.c_int64
     L.mh_eval.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_int64] * 9
