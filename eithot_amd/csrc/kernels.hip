@@ -1744,8 +1744,9 @@ __device__ __forceinline__ uint64_t seg_addr(const SegArgs& a, uint32_t s, uint3
   return addr;
 }
 
-// f0: first matrix of the workgroup's K
-template <typename Word, int V>
+// f0: first matrix of the workgroup's K; NT: non-temporal global loads (bit
+// 0) and stores (bit 1)
+template <typename Word, int V, int NT = 0>
 __device__ __forceinline__ void seg_body(const SegArgs& a, const uint32_t f0, unsigned char* lds) {
   constexpr uint32_t w = sizeof(Word);
   const uint32_t nf = (a.B - f0) < a.K ? (a.B - f0) : a.K;
@@ -1765,8 +1766,7 @@ __device__ __forceinline__ void seg_body(const SegArgs& a, const uint32_t f0, un
       if (t < n) {
         const uint32_t k = a.fwpm.div(t), q = t - k * a.wpm;  // word q of matrix k
         const uint32_t s = a.fwps.div(q);
-        v[u] = *reinterpret_cast<const Word*>(a.data + tbl[k * a.nseg + s] +
-                                              (uint64_t)(q - s * a.fwps.d) * w);
+        v[u] = ld_nt<NT, Word>(a.data + tbl[k * a.nseg + s] + (uint64_t)(q - s * a.fwps.d) * w);
       }
     }
 #pragma unroll
@@ -1796,14 +1796,14 @@ __device__ __forceinline__ void seg_body(const SegArgs& a, const uint32_t f0, un
       val = lds_getw<Word>(l + i * a.rs + (x - i * a.fwpe.d) * w);
     }
     const uint32_t s = a.fwps.div(q);
-    *reinterpret_cast<Word*>(a.data + tbl[k * a.nseg + s] + (uint64_t)(q - s * a.fwps.d) * w) = val;
+    st_nt<NT, Word>(a.data + tbl[k * a.nseg + s] + (uint64_t)(q - s * a.fwps.d) * w, val);
   }
 }
 
-template <typename Word, int V>
+template <typename Word, int V, int NT = 0>
 __global__ __launch_bounds__(512) void k_segtile(SegArgs a, int remap) {
   extern __shared__ __align__(16) unsigned char lds[];
-  seg_body<Word, V>(a, xcd_logical(blockIdx.x, gridDim.x, remap) * a.K, lds);
+  seg_body<Word, V, NT>(a, xcd_logical(blockIdx.x, gridDim.x, remap) * a.K, lds);
 }
 
 // Workgroup size of a segmented tile pass: enough threads that each issues
@@ -1903,6 +1903,18 @@ template <typename Word, int V>
 hipError_t launch_seg_wv(const SegArgs& a, int64_t lds, int64_t groups, const LaunchCfg& cfg,
                          hipStream_t stream) {
   const int threads = segtile_threads(a, cfg);
+  if constexpr (sizeof(Word) == 16) {
+    if (cfg.segtile_nt == 3) {
+      hipLaunchKernelGGL((k_segtile<Word, V, 3>), dim3((uint32_t)groups), dim3(threads),
+                         (size_t)lds, stream, a, cfg.xcd_remap);
+      return hipGetLastError();
+    }
+    if (cfg.segtile_nt == 2) {
+      hipLaunchKernelGGL((k_segtile<Word, V, 2>), dim3((uint32_t)groups), dim3(threads),
+                         (size_t)lds, stream, a, cfg.xcd_remap);
+      return hipGetLastError();
+    }
+  }
   hipLaunchKernelGGL((k_segtile<Word, V>), dim3((uint32_t)groups), dim3(threads), (size_t)lds,
                      stream, a, cfg.xcd_remap);
   return hipGetLastError();
@@ -2414,6 +2426,18 @@ void cfg_word(int64_t bytes, hipError_t& e) {
   if (w >= 4) set((const void*)k_segtile<Word, (w >= 4 ? 4 : 1)>);
   if (w >= 8) set((const void*)k_segtile<Word, (w >= 8 ? 8 : 1)>);
   if (w >= 16) set((const void*)k_segtile<Word, (w >= 16 ? 16 : 1)>);
+  if constexpr (w == 16) {
+    set((const void*)k_segtile<Word, 1, 2>);
+    set((const void*)k_segtile<Word, 2, 2>);
+    set((const void*)k_segtile<Word, 4, 2>);
+    set((const void*)k_segtile<Word, 8, 2>);
+    set((const void*)k_segtile<Word, 16, 2>);
+    set((const void*)k_segtile<Word, 1, 3>);
+    set((const void*)k_segtile<Word, 2, 3>);
+    set((const void*)k_segtile<Word, 4, 3>);
+    set((const void*)k_segtile<Word, 8, 3>);
+    set((const void*)k_segtile<Word, 16, 3>);
+  }
 }
 }  // namespace
 

@@ -902,6 +902,7 @@ static int64_t* tuning_field(Tuning& t, const char* key) {
   if (!strcmp(key, "cycle_cell_pref")) return &t.cycle_cell_pref;
   if (!strcmp(key, "segtile_threads")) return &t.segtile_threads;
   if (!strcmp(key, "lds_threads_auto")) return &t.lds_threads_auto;
+  if (!strcmp(key, "segtile_nt")) return &t.segtile_nt;
   return nullptr;
 }
 

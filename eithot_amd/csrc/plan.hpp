@@ -186,6 +186,7 @@ struct Tuning {
   int64_t cycle_cell_pref = 0;
   int64_t segtile_threads = 0;
   int64_t lds_threads_auto = 1;
+  int64_t segtile_nt = 3;
   uint64_t epoch = 0;  // bumped on every change (plan caches key on it)
 };
 Tuning tuning_get();

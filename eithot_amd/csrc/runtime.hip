@@ -92,6 +92,7 @@ LaunchCfg Device::launch_cfg() const {
   c.cycle_variant = (int)t.cycle_variant;
   c.segtile_threads = (int)t.segtile_threads;
   c.lds_threads_auto = (int)t.lds_threads_auto;
+  c.segtile_nt = (int)t.segtile_nt;
   return c;
 }
 
