@@ -153,13 +153,33 @@ int Device::release_arena_after(hipStream_t stream) {
   return EITHOT_OK;
 }
 
+// Timing events of this thread on each device, reused from call to call
+// (creating and destroying them per call costs more than the launches).
+std::vector<hipEvent_t>& timing_events(size_t n) {
+  thread_local std::map<int, std::vector<hipEvent_t>> pools;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::vector<hipEvent_t>& v = pools[dev];
+  while (v.size() < n) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) break;
+    v.push_back(e);
+  }
+  return v;
+}
+
 int run_plan(Device* d, const Plan& plan, char* data, char* scratch, int64_t scratch_bytes,
              hipStream_t stream, std::vector<float>* times) {
   std::vector<hipEvent_t> ev;
   if (times) {
-    ev.resize(plan.ops.size() + 1);
-    for (auto& e : ev) hipEventCreate(&e);
-    hipEventRecord(ev[0], stream);
+    const std::vector<hipEvent_t>& pool = timing_events(plan.ops.size() + 1);
+    if (pool.size() >= plan.ops.size() + 1) {
+      ev.assign(pool.begin(), pool.begin() + (long)(plan.ops.size() + 1));
+      hipEventRecord(ev[0], stream);
+    } else {
+      times->clear();
+      times = nullptr;  // no events to time with
+    }
   }
   int rc = EITHOT_OK;
   const LaunchCfg cfg = d->launch_cfg();
@@ -181,7 +201,6 @@ int run_plan(Device* d, const Plan& plan, char* data, char* scratch, int64_t scr
         times->push_back(ms);
       }
     }
-    for (auto& e : ev) hipEventDestroy(e);
   }
   return rc;
 }
