@@ -918,6 +918,10 @@ int tuning_set(const char* key, int64_t value) {
   if (f == &t.cycle_variant && value > 7) return EITHOT_ERR_INVALID;
   if (f == &t.cycle_lmax && value && (value < 2 || value > (1ll << 20))) return EITHOT_ERR_INVALID;
   if (f == &t.lds_target && value < 1024) return EITHOT_ERR_INVALID;
+  if (f == &t.segtile_nt && value != 0 && value != 2 && value != 3) return EITHOT_ERR_INVALID;
+  const bool flag = f == &t.regroup || f == &t.lds_threads_auto || f == &t.cycle_cell_pref ||
+                    f == &t.cycle_word_cost || f == &t.block_digits || f == &t.xcd_remap;
+  if (flag && value > 1) return EITHOT_ERR_INVALID;
   *f = value;
   t.epoch = g_tune.epoch + 1;
   g_tune = t;

@@ -183,8 +183,8 @@ int eithot_plan_segments(const int64_t* dims, const int* perm, int order, size_t
  *   "threads", "threads_big"  workgroup sizes (256, 512; multiples of 64, at most 512)
  *   "segtile_threads" workgroup size of segmented tile passes (0: one round of
  *                    8 loads per thread, at most 512; else a multiple of 64)
- *   "segtile_nt"     non-temporal loads (bit 0) and stores (bit 1) in segmented
- *                    tile passes of 16-byte words (3)
+ *   "segtile_nt"     segmented tile passes of 16-byte words: 3 non-temporal loads
+ *                    and stores (default), 2 non-temporal stores, 0 neither
  *   "lds_threads_auto" tile, column, row and block workgroups sized for one round of
  *                    8 loads per thread (1) or from "threads"/"threads_big" (0)
  *   "xcd_remap"      keep consecutive footprints on one XCD (1)

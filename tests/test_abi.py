@@ -5,6 +5,8 @@ import os
 import re
 import subprocess
 
+import pytest
+
 import eithot_amd as E
 from eithot_amd import _lib
 
@@ -43,3 +45,19 @@ def test_error_message_thread_local_and_planning_without_gpu():
     rc = E.lib().eithot_transpose(None, E.i64([2]), E.i32([1]), 1, 4, 1, 0.05)
     assert rc == -1
     assert b"null" in E.lib().eithot_last_error()
+
+
+def test_tuning_rejects_out_of_range_values():
+    import eithot_amd as E
+    for key, bad in (("segtile_nt", 1), ("regroup", 2), ("lds_threads_auto", 5),
+                     ("cycle_variant", 8), ("threads", 96), ("cycle_lmax", 1)):
+        with pytest.raises(E.EithotError):
+            E.set_tuning(key, bad)
+    for key, good in (("segtile_nt", 2), ("regroup", 0), ("cycle_lmax", 64)):
+        old = E.set_tuning(key, good)
+        try:
+            assert E.get_tuning(key) == good
+        finally:
+            E.set_tuning(key, old)
+    with pytest.raises(E.EithotError):
+        E.set_tuning("no_such_knob", 1)
