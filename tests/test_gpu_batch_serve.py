@@ -75,6 +75,14 @@ def test_server_alone_and_in_the_mix(torch_cuda):
     torch = torch_cuda
     rng = np.random.default_rng(4)
     specs = [random_tensor(rng, orders=(3, 6), dmax=200, vmax=2_000_000) for _ in range(40)]
+    # plans with a segmented tile (a composed cycle pass around it), tile
+    # last and tile first, and a fused block pass
+    specs += [([12, 12, 12, 16, 20], [1, 5, 3, 2, 4], 4), ([2, 32, 4, 5, 32], [5, 2, 4, 1, 3], 4),
+              ([48, 40, 8, 30], [3, 2, 1, 4], 4)]
+    kinds = set()
+    for dims, perm, elem in specs[-3:]:
+        kinds |= {o["kind"] for o in E.plan_describe(dims, perm, elem, 1, 0.05)[0]}
+    assert 6 in kinds, kinds
     # alone: one request at a time through the plain entry point
     alone = []
     for k, (dims, perm, elem) in enumerate(specs):
