@@ -7,7 +7,7 @@ libeithot.so:
   LARGE_RATIO)`` -- the reference library call ``inplace::transpose``
   (ITNT/include/transpose.h:8): 0-based permutation, element offset
   ``source``, ``dim[0]`` fastest.
-* ``transpose_tensor(t, dims, perm1, ...)`` -- the north-star C-ABI
+* ``transpose_tensor(t, dims, perm1, ...)`` -- the main C-ABI entry point
   ``eithot_transpose`` on a device tensor, 1-based permutation.
 * ``estimate(...)`` / ``estimate_reference(...)`` -- the Nd_estimation planner.
 * ``Batch`` and ``Server`` -- batched and served transposition.
