@@ -451,7 +451,7 @@ def main():
     d = agg[dom]
     achieved = d["bytes"] / (d["ms"] / 1e3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "config2_traffic.json")
+    tf = os.path.join(ROOT, "profiles", "traffic_6d_4.5GB.json")
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get(dom)
 

@@ -9,10 +9,10 @@ from gpu_util import assert_bitexact, run_case
 pytestmark = pytest.mark.gpu
 
 
-def test_config1_known_answer(torch_cuda):
+def test_known_answer_2x4096x1024x2(torch_cuda):
     # (2,4096,1024,2) perm 1 3 2 4, v(i) = i: the reference's own check
     got, want, peak = run_case(torch_cuda, [2, 4096, 1024, 2], [1, 3, 2, 4], 4, mode=0)
-    assert_bitexact(got, want, "config 1")
+    assert_bitexact(got, want, "(2,4096,1024,2) perm 1 3 2 4")
     assert peak <= 0.05 * got.nbytes
 
 

@@ -234,8 +234,8 @@ def test_partition_plan_executes_correctly():
         assert np.array_equal(b, O.permute(a, dims, [p - 1 for p in perm1], 4))
 
 
-def test_config_plans():
-    # config 1: at most the three Catanzaro passes of the reference's lowering,
+def test_benchmark_shape_plans():
+    # (2,4096,1024,2) perm 1 3 2 4: at most the three Catanzaro passes of the reference's lowering,
     # scratch within 5 % of 64 MiB
     ops, scratch = E.plan_describe([2, 4096, 1024, 2], [1, 3, 2, 4], 4, 1, 0.05)
     assert len(ops) <= 3 and not any(o["staged"] for o in ops)
@@ -250,13 +250,13 @@ def test_config_plans():
             E.set_tuning(k, v)
     assert all(o["kind"] in (1, 2, 3) and not o["staged"] for o in ops), ops
     assert scratch == 0
-    # config 2: two passes (a composed cycle pass and a segmented tile)
+    # (32,48,64,40,36,8) perm 6 4 2 5 1 3: two passes (a composed cycle pass and a segmented tile)
     # where the reference runs five stages, none staged, scratch within 5 %
     # of 4.53 GB
     ops, scratch = E.plan_describe([32, 48, 64, 40, 36, 8], [6, 4, 2, 5, 1, 3], 4, 1, 0.05)
     assert len(ops) <= 2 and all(o["kind"] in (1, 4, 5, 6) and not o["staged"] for o in ops)
     assert scratch <= 0.05 * 32 * 48 * 64 * 40 * 36 * 8 * 4
-    # config 3: scratch within 5 % of 171.8 GB
+    # (2,163840,65536,2) perm 1 3 2 4: scratch within 5 % of 171.8 GB
     ops, scratch = E.plan_describe([2, 163840, 65536, 2], [1, 3, 2, 4], 4, 1, 0.05)
     assert 0 < scratch <= 0.05 * 2 * 163840 * 65536 * 2 * 4
 
