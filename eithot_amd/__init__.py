@@ -21,7 +21,7 @@ from . import _lib
 from ._lib import EithotError, check, i32, i64, lib
 
 __all__ = ["transpose", "transpose_tensor", "transpose_ptr", "estimate", "estimate_reference",
-           "decompose", "plan_describe", "last_peak_extra_bytes", "fill_pattern",
+           "decompose", "plan_describe", "plan_partition", "last_peak_extra_bytes", "fill_pattern",
            "verify_pattern", "set_tuning", "get_tuning", "Batch", "Server", "EithotError"]
 
 
@@ -143,6 +143,18 @@ def plan_describe(dims, perm1, elem_bytes, n_sub=1, alpha=0.05, lds_max=0):
             d["outer"] = [(outer[2 * q], outer[2 * q + 1]) for q in range(k // 16)]
         out.append(d)
     return out, scratch.value
+
+
+def plan_partition(dims, perm1, elem_bytes, n_sub, alpha=0.05):
+    """Sub-tensor partition of the plan: (count, 0-based dim, part_len, last_len)."""
+    n = ctypes.c_int()
+    d = ctypes.c_int()
+    q = ctypes.c_int64()
+    r = ctypes.c_int64()
+    check(lib().eithot_plan_partition(i64(dims), i32(perm1), len(dims), int(elem_bytes),
+                                      int(n_sub), float(alpha), ctypes.byref(n), ctypes.byref(d),
+                                      ctypes.byref(q), ctypes.byref(r)))
+    return n.value, d.value, q.value, r.value
 
 
 def set_tuning(key, value):

@@ -27,7 +27,8 @@ EXPORTS = [
     "eithot_transpose", "eithot_transpose_async", "eithot_inplace_transpose",
     "eithot_last_peak_extra_bytes", "eithot_last_error", "eithot_release", "eithot_estimate",
     "eithot_estimate_reference", "eithot_decompose", "eithot_plan_describe",
-    "eithot_plan_block_table", "eithot_plan_cycle_maps", "eithot_plan_segments",
+    "eithot_plan_partition", "eithot_plan_block_table", "eithot_plan_cycle_maps",
+    "eithot_plan_segments",
     "eithot_set_tuning",
     "eithot_get_tuning",
     "eithot_set_pass_timing", "eithot_last_pass_times", "eithot_fill_pattern",
@@ -90,6 +91,8 @@ def lib():
                                    ctypes.c_int]
     L.eithot_plan_describe.argtypes = [i64p, ip, ctypes.c_int, sz, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_int64, ctypes.POINTER(OpDesc), ctypes.c_int, i64p]
+    L.eithot_plan_partition.argtypes = [i64p, ip, ctypes.c_int, sz, ctypes.c_int,
+                                        ctypes.c_double, ip, ip, i64p, i64p]
     L.eithot_plan_block_table.restype = ctypes.c_int64
     L.eithot_plan_block_table.argtypes = [i64p, ip, ctypes.c_int, sz, ctypes.c_int, ctypes.c_double,
                                           ctypes.c_int64, ctypes.c_int,

@@ -1274,23 +1274,37 @@ __global__ __launch_bounds__(256) void k_cycle_pipe(CycArgs a) {
   }
 }
 
+// Owner of a device allocation on device `dev`: its deleter waits for that
+// device to go idle before freeing, so a pass enqueued earlier never reads
+// freed memory.  The caller's current device is restored.
+std::shared_ptr<void> device_owned(void* p, int dev) {
+  return std::shared_ptr<void>(p, [dev](void* q) {
+    int cur = 0;
+    const bool have = hipGetDevice(&cur) == hipSuccess;
+    if (have && cur != dev) (void)hipSetDevice(dev);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(q);
+    if (have && cur != dev) (void)hipSetDevice(cur);
+  });
+}
+
 const void* cycle_table_device(const CycleTable& t, hipError_t* err) {
   *err = hipSuccess;
   int dev = 0;
   if ((*err = hipGetDevice(&dev)) != hipSuccess) return nullptr;
-  std::lock_guard<std::mutex> lk(t.dev_mu);
-  auto it = t.dev.find(dev);
-  if (it != t.dev.end()) return it->second.get();
+  std::lock_guard<std::mutex> lk(t.dev.mu);
+  auto it = t.dev.by_dev.find(dev);
+  if (it != t.dev.by_dev.end()) return it->second.get();
   const size_t sb = t.segs.size() * sizeof(CycleSeg), lb = t.slots.size() * sizeof(uint32_t);
   void* p = nullptr;
   if ((*err = hipMalloc(&p, std::max<size_t>(sb + lb, 16))) != hipSuccess) return nullptr;
-  std::shared_ptr<void> own(p, [](void* q) { (void)hipFree(q); });
+  std::shared_ptr<void> own = device_owned(p, dev);
   if (sb && (*err = hipMemcpy(p, t.segs.data(), sb, hipMemcpyHostToDevice)) != hipSuccess)
     return nullptr;
   if (lb && (*err = hipMemcpy((char*)p + sb, t.slots.data(), lb, hipMemcpyHostToDevice)) !=
                 hipSuccess)
     return nullptr;
-  t.dev[dev] = own;
+  t.dev.by_dev[dev] = own;
   return p;
 }
 
@@ -1534,20 +1548,20 @@ const void* block_table_device(const BlockTable& t, hipError_t* err) {
   *err = hipSuccess;
   int dev = 0;
   if ((*err = hipGetDevice(&dev)) != hipSuccess) return nullptr;
-  std::lock_guard<std::mutex> lk(t.dev_mu);
-  auto it = t.dev.find(dev);
-  if (it != t.dev.end()) return it->second.get();
+  std::lock_guard<std::mutex> lk(t.dev.mu);
+  auto it = t.dev.by_dev.find(dev);
+  if (it != t.dev.by_dev.end()) return it->second.get();
   const bool i16 = t.cells <= 65536;
   std::vector<uint16_t> h16;
   if (i16) h16.assign(t.src.begin(), t.src.end());
   const size_t nb = t.src.size() * (i16 ? 2 : 4);
   void* p = nullptr;
   if ((*err = hipMalloc(&p, std::max<size_t>(nb, 16))) != hipSuccess) return nullptr;
-  std::shared_ptr<void> own(p, [](void* q) { (void)hipFree(q); });
+  std::shared_ptr<void> own = device_owned(p, dev);
   if ((*err = hipMemcpy(p, i16 ? (const void*)h16.data() : (const void*)t.src.data(), nb,
                         hipMemcpyHostToDevice)) != hipSuccess)
     return nullptr;
-  t.dev[dev] = own;
+  t.dev.by_dev[dev] = own;
   return p;
 }
 

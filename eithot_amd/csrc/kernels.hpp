@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <utility>
 #include <vector>
 
@@ -31,6 +32,10 @@ hipError_t launch_op(const Op& op, char* data, char* scratch, int64_t scratch_by
 // (the cycle table of OP_CYCLE; nothing for other kinds).  Call before
 // capturing the op into a graph.
 hipError_t prepare_op(const Op& op);
+
+// Owner of device memory `p` on device `dev` whose deleter waits for the
+// device to go idle before hipFree.
+std::shared_ptr<void> device_owned(void* p, int dev);
 
 // Bytes of LDS the op's LDS kernel uses per workgroup with cfg (0 if staged).
 int64_t op_lds_bytes(const Op& op, const LaunchCfg& cfg);

@@ -694,7 +694,8 @@ struct Lowerer {
     auto key = std::make_tuple(R, C, U, 0);  // first result wins at any depth
     auto it = memo.find(key);
     if (it != memo.end()) return it->second;
-    if (int64_t us = pick_tile_slice(R, C, U, prof)) {
+    const int64_t mask = prof.lower_mask;
+    if (int64_t us = (mask & LOWER_TILE) ? pick_tile_slice(R, C, U, prof) : 0) {
       ch.how = HOW_TILE;
       ch.cost = us == U ? kCostTile : kCostTileSliced;
       if (us == U) {
@@ -703,19 +704,19 @@ struct Lowerer {
       }
     }
     CatPlan cp = plan_catanzaro(R, C, U, prof, scratch_budget);
-    if (cp.ok && cp.cost < ch.cost) {
+    if ((mask & LOWER_CATANZARO) && cp.ok && cp.cost < ch.cost) {
       ch.how = HOW_CAT;
       ch.cost = cp.cost;
     }
     // whole cells along the cycles of the transpose, no LDS
-    if (prof.cycle_min_cell > 0 && U >= prof.cycle_min_cell &&
+    if ((mask & LOWER_CYCLE) && prof.cycle_min_cell > 0 && U >= prof.cycle_min_cell &&
         (uint64_t)R * (uint64_t)C <= kCycleMaxCells &&
         cycle_cost(U) < ch.cost) {
       ch.how = HOW_CYCLE;
       ch.cost = cycle_cost(U);
     }
     // whole matrices (or slices) through scratch: copy out, tiled transpose back
-    if (ch.cost > kCostTileStaged * prof.staged_mult) {
+    if ((mask & LOWER_STAGED_TILE) && ch.cost > kCostTileStaged * prof.staged_mult) {
       const int64_t us = pick_staged_tile_slice(R, C, U, scratch_budget, prof);
       const double cost = staged_tile_cost(R, U, us) * prof.staged_mult;
       if (us && cost < ch.cost) {
@@ -723,7 +724,7 @@ struct Lowerer {
         ch.cost = cost;
       }
     }
-    if (depth < 3 && ch.cost > 2 * kCostTile + 1e-9) {
+    if ((mask & LOWER_SPLIT) && depth < 3 && ch.cost > 2 * kCostTile + 1e-9) {
       // R = R1 * R0: (B*R1, R0, C, U) then (B, R1, C, R0*U)
       for (int64_t r0 : divisors(R)) {
         if (r0 <= 1 || r0 >= R) continue;
@@ -903,6 +904,7 @@ static int64_t* tuning_field(Tuning& t, const char* key) {
   if (!strcmp(key, "segtile_threads")) return &t.segtile_threads;
   if (!strcmp(key, "lds_threads_auto")) return &t.lds_threads_auto;
   if (!strcmp(key, "segtile_nt")) return &t.segtile_nt;
+  if (!strcmp(key, "lower_mask")) return &t.lower_mask;
   return nullptr;
 }
 
@@ -919,6 +921,7 @@ int tuning_set(const char* key, int64_t value) {
   if (f == &t.cycle_lmax && value && (value < 2 || value > (1ll << 20))) return EITHOT_ERR_INVALID;
   if (f == &t.lds_target && value < 1024) return EITHOT_ERR_INVALID;
   if (f == &t.segtile_nt && value != 0 && value != 2 && value != 3) return EITHOT_ERR_INVALID;
+  if (f == &t.lower_mask && (value == 0 || value > LOWER_ALL)) return EITHOT_ERR_INVALID;
   const bool flag = f == &t.regroup || f == &t.lds_threads_auto || f == &t.cycle_cell_pref ||
                     f == &t.cycle_word_cost || f == &t.block_digits || f == &t.xcd_remap;
   if (flag && value > 1) return EITHOT_ERR_INVALID;
@@ -950,6 +953,7 @@ void tuning_apply(const Tuning& t, DeviceProfile& prof) {
   prof.regroup = t.regroup;
   prof.cycle_lmax = t.cycle_lmax;
   prof.cycle_cell_pref = t.cycle_cell_pref;
+  prof.lower_mask = t.lower_mask;
 }
 
 void lower_transpose2d(int64_t B, int64_t R, int64_t C, int64_t U, uint64_t base, int stage,
@@ -1606,38 +1610,72 @@ static int make_plan_impl(const int64_t* dims_in, const int* perm1, int rank, si
   for (int i = 1; i < rank; ++i)
     if (dims[i] > dims[p]) p = i;
   const int64_t large = (int64_t)(32.0 * (1.0 / alpha));
-  if (n_sub > 1 && dims[p] > large && !stages.empty()) {
-    // the largest divisor of the split dim not above n_sub: every sub-tensor
-    // has the same shape, so no padding headroom is needed in the buffer
-    for (int q = n_sub; q >= 2; --q)
-      if (dims[p] % q == 0) { nb = q; break; }
-  }
+  if (n_sub > 1 && dims[p] > large && !stages.empty())
+    nb = (int)std::min<int64_t>(n_sub, dims[p]);
   plan.n_sub = nb;
   if (nb > 1) {
+    // The split dim D becomes nb - 1 parts of q indices and a last part of
+    // r (r == q when nb divides D).  The reference pads D up to a multiple
+    // of nb instead, which grows the tensor (Nd_padding.cu:102-223); a
+    // ragged last sub-tensor keeps the caller's buffer exactly the tensor.
     plan.partition_dim = p;
-    int64_t ds = 1, nds = 1;
-    for (int i = 0; i <= p; ++i) ds *= dims[i];
-    for (int i = p + 1; i < rank; ++i) nds *= dims[i];
-    // partition (Nd_decompose.cpp:16-25,48-56): (ds/nb, nb, nds) -> 132
-    L.emit(1, nds, nb, (ds / nb) * (int64_t)elem, 0, -1, 0, plan.ops);
-    std::vector<int64_t> sub = dims;
-    sub[p] /= nb;
-    std::vector<Stage> sst;
-    decompose(sub.data(), perm1, rank, reorder, sst);
-    for (size_t s = 0; s < sst.size(); ++s) {
-      const Stage& st = sst[s];
-      // all nb sub-tensors are contiguous and share the plan: one batched op
-      L.emit(st.S * nb, st.Y, st.X, st.P * (int64_t)elem, 0, (int)s, 0, plan.ops);
-    }
-    plan.stages = sst;
-    // join (Nd_decompose.cpp:32-41,65-82): (ds'/nb, nds', nb) -> 132
-    int pp = 0;
+    const int64_t D = dims[p], E = (int64_t)elem;
+    int64_t q = (D + nb - 1) / nb;
+    if ((nb - 1) * q >= D) q = D / nb;
+    const int64_t r = D - (nb - 1) * q;
+    plan.part_len = q;
+    plan.last_len = r;
+    int64_t A = 1, S = 1;  // input: (A, D, S), fastest first
+    for (int i = 0; i < p; ++i) A *= dims[i];
+    for (int i = p + 1; i < rank; ++i) S *= dims[i];
+    int pp = 0;  // output position of the split dim: (A2, D, S2)
     for (int i = 0; i < rank; ++i)
       if (perm1[i] - 1 == p) pp = i;
-    int64_t ds2 = 1, nds2 = 1;
-    for (int i = 0; i <= pp; ++i) ds2 *= dims[perm1[i] - 1];
-    for (int i = pp + 1; i < rank; ++i) nds2 *= dims[perm1[i] - 1];
-    L.emit(1, nb, nds2, (ds2 / nb) * (int64_t)elem, 0, -1, 0, plan.ops);
+    int64_t A2 = 1, S2 = 1;
+    for (int i = 0; i < pp; ++i) A2 *= dims[perm1[i] - 1];
+    for (int i = pp + 1; i < rank; ++i) S2 *= dims[perm1[i] - 1];
+    std::vector<int64_t> subq = dims, subr = dims;
+    subq[p] = q;
+    subr[p] = r;
+    std::vector<Stage> sq, sr;
+    decompose(subq.data(), perm1, rank, reorder, sq);
+    decompose(subr.data(), perm1, rank, reorder, sr);
+    plan.stages = sq;
+    const int neq = r == q ? nb : nb - 1;  // sub-tensors of q indices
+    if (r != q) {
+      // Every slab s of the input is [X_s | Y_s] with X_s the first
+      // (nb-1)*q indices of D and Y_s the last r: gather all X slabs before
+      // all Y slabs.  In cells of g = gcd(X, Y) elements, that is the S x
+      // (x+y) transpose followed by the x x S and y x S transposes of the
+      // two regions it leaves.
+      const int64_t X = A * q * (nb - 1), Y = A * r, g = gcd64(X, Y);
+      const int64_t x = X / g, y = Y / g;
+      L.emit(1, S, x + y, g * E, 0, -1, 0, plan.ops);
+      L.emit(1, x, S, g * E, 0, -1, 0, plan.ops);
+      L.emit(1, y, S, g * E, (uint64_t)(x * S * g * E), -1, 0, plan.ops);
+    }
+    // partition (Nd_decompose.cpp:16-25,48-56): (A*q, neq, S) -> 132, so
+    // the neq sub-tensors of q indices are contiguous; the last one (when
+    // ragged) already is, after them
+    L.emit(1, S, neq, A * q * E, 0, -1, 0, plan.ops);
+    // the sub-tensors of q indices share one batched plan
+    for (size_t st = 0; st < sq.size(); ++st)
+      L.emit(sq[st].S * neq, sq[st].Y, sq[st].X, sq[st].P * E, 0, (int)st, 0, plan.ops);
+    const uint64_t last = (uint64_t)(A * q * neq * S * E);
+    if (r != q)
+      for (size_t st = 0; st < sr.size(); ++st)
+        L.emit(sr[st].S, sr[st].Y, sr[st].X, sr[st].P * E, last, (int)st, 0, plan.ops);
+    // join (Nd_decompose.cpp:32-41,65-82): (A2*q, S2, neq) -> 132
+    L.emit(1, neq, S2, A2 * q * E, 0, -1, 0, plan.ops);
+    if (r != q) {
+      // the inverse of the gather above on the output side: slabs [X2_s]
+      // then [Y2_s] back to [X2_s | Y2_s]
+      const int64_t X2 = A2 * q * (nb - 1), Y2 = A2 * r, g2 = gcd64(X2, Y2);
+      const int64_t x2 = X2 / g2, y2 = Y2 / g2;
+      L.emit(1, S2, x2, g2 * E, 0, -1, 0, plan.ops);
+      L.emit(1, S2, y2, g2 * E, (uint64_t)(x2 * S2 * g2 * E), -1, 0, plan.ops);
+      L.emit(1, x2 + y2, S2, g2 * E, 0, -1, 0, plan.ops);
+    }
   } else {
     for (size_t s = 0; s < stages.size(); ++s) {
       const Stage& st = stages[s];
@@ -1816,6 +1854,28 @@ static int make_plan_search(const int64_t* dims_in, const int* perm1, int rank, 
   }
   plan = best;
   return EITHOT_OK;
+}
+
+int64_t Plan::device_table_bytes(int dev) const {
+  int64_t n = 0;
+  std::vector<const void*> seen;  // a table shared by several passes counts once
+  auto add = [&](const void* t, const DeviceCopies& c, int64_t bytes) {
+    if (std::find(seen.begin(), seen.end(), t) != seen.end() || !c.has(dev)) return;
+    seen.push_back(t);
+    n += std::max<int64_t>(bytes, 16);  // the uploads allocate at least 16 bytes
+  };
+  for (const Op& op : ops) {
+    if (op.cyc) add(op.cyc.get(), op.cyc->dev, op.cyc->bytes());
+    if (op.blk) add(op.blk.get(), op.blk->dev, op.blk->bytes());
+  }
+  return n;
+}
+
+void Plan::drop_device_tables(int dev) const {
+  for (const Op& op : ops) {
+    if (op.cyc) op.cyc->dev.drop(dev);
+    if (op.blk) op.blk->dev.drop(dev);
+  }
 }
 
 std::string Plan::describe() const {

@@ -48,6 +48,30 @@ enum MapKind : int {
   MAP_ROWD = 6,        // R2C row shuffle (gather form)
 };
 
+// Device copies of a host table, one per device id, made by the launcher on
+// first use.  Each copy is a shared_ptr whose deleter waits for the device
+// to go idle before freeing it, so dropping a copy never frees memory that a
+// pass already enqueued still reads.
+struct DeviceCopies {
+  mutable std::mutex mu;
+  mutable std::map<int, std::shared_ptr<void>> by_dev;
+  bool has(int dev) const {
+    std::lock_guard<std::mutex> lk(mu);
+    return by_dev.count(dev) != 0;
+  }
+  // Drops the copy on `dev`; it is freed when the last holder lets go.
+  void drop(int dev) const {
+    std::shared_ptr<void> hold;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      auto it = by_dev.find(dev);
+      if (it == by_dev.end()) return;
+      hold = std::move(it->second);
+      by_dev.erase(it);
+    }
+  }
+};
+
 // Cycle structure of the R x C transpose of cells (OP_CYCLE).  In gather
 // form output position p = j*R + i takes input position src(p) = i*C + j;
 // following p -> src(p) walks a cycle.  Cycles longer than `lmax` are cut
@@ -77,8 +101,7 @@ struct CycleTable {
   std::vector<uint32_t> slots;  // slot -> position whose old cell it holds
   int64_t bytes() const { return (int64_t)(segs.size() * sizeof(CycleSeg) + slots.size() * 4); }
   // device copies (segs then slots), one per device id, made by the launcher
-  mutable std::mutex dev_mu;
-  mutable std::map<int, std::shared_ptr<void>> dev;
+  DeviceCopies dev;
 };
 // Builds the table (fixed points are skipped).  R*C must be below 2^32.
 std::shared_ptr<CycleTable> build_cycle_table(int64_t R, int64_t C, int64_t lmax);
@@ -105,8 +128,7 @@ struct BlockTable {
   }
   // device bytes: the gather table, unless the digit form replaces it
   int64_t bytes() const { return ndig ? 0 : (int64_t)src.size() * (cells <= 65536 ? 2 : 4); }
-  mutable std::mutex dev_mu;
-  mutable std::map<int, std::shared_ptr<void>> dev;  // device copy (uint16 or uint32 indices)
+  DeviceCopies dev;  // device copy (uint16 or uint32 indices), only in table form
 };
 
 struct Op {
@@ -143,6 +165,16 @@ struct Op {
   uint64_t bytes() const { return (uint64_t)B * R * C * U; }
 };
 
+// Lowering options of one batched 2-D transpose (the "lower_mask" knob).
+enum LowerOption : int64_t {
+  LOWER_TILE = 1,         // LDS tile, whole units or slices
+  LOWER_CATANZARO = 2,    // Catanzaro column / row passes (LDS or staged)
+  LOWER_SPLIT = 4,        // hierarchical split into two smaller transposes
+  LOWER_STAGED_TILE = 8,  // tile through scratch
+  LOWER_CYCLE = 16,       // cycle following of whole cells
+  LOWER_ALL = 31
+};
+
 struct DeviceProfile {
   int64_t lds_max = 160 * 1024;   // bytes of LDS one workgroup may use (gfx950)
   int64_t lds_target = 64 * 1024; // per-workgroup LDS the launcher packs footprints to
@@ -162,6 +194,7 @@ struct DeviceProfile {
   int64_t regroup = 1;            // with reorder 0: also plan cell pass / tile / cell pass
   int64_t cycle_lmax = 0;         // > 0: cycle segments of at most this many positions
   int64_t cycle_cell_pref = 0;    // 1: cycle passes of larger cells cost less
+  int64_t lower_mask = LOWER_ALL; // lowering options the planner may use
   double staged_mult = 1.0;       // planner-internal: weight of staged passes in the search
 };
 
@@ -187,6 +220,7 @@ struct Tuning {
   int64_t segtile_threads = 0;
   int64_t lds_threads_auto = 1;
   int64_t segtile_nt = 3;
+  int64_t lower_mask = LOWER_ALL;
   uint64_t epoch = 0;  // bumped on every change (plan caches key on it)
 };
 Tuning tuning_get();
@@ -199,16 +233,23 @@ struct Plan {
   std::vector<Op> ops;
   int n_sub = 1;              // sub-tensors actually used
   int partition_dim = -1;     // 0-based input dim split into n_sub blocks
+  int64_t part_len = 0;       // indices of the split dim in each of the first n_sub - 1
+  int64_t last_len = 0;       // indices in the last sub-tensor (== part_len unless ragged)
   int64_t scratch_bytes = 0;  // device scratch the plan needs (0 if all LDS)
   int64_t table_bytes = 0;    // device cycle tables of its OP_CYCLE passes
   uint64_t tensor_bytes = 0;
   std::string describe() const;
+  // Bytes of this plan's pass tables that are resident on device `dev`.
+  int64_t device_table_bytes(int dev) const;
+  // Frees this plan's pass tables on device `dev` (uploaded again on next use).
+  void drop_device_tables(int dev) const;
 };
 
 // Full plan.  perm1 is 1-based.  alpha bounds the scratch (scratch <= alpha *
 // tensor bytes, at least one footprint); n_sub > 1 requests the reference's
 // sub-tensor partitioning along the largest dim when that dim exceeds
-// floor(32/alpha) and divides evenly.  reorder: 0 plans both decomposer
+// floor(32/alpha): min(n_sub, dim) sub-tensors, the last one ragged when the
+// count does not divide the dim.  reorder: 0 plans both decomposer
 // orders and keeps the cheaper (the reference's mix rule breaks ties), 1
 // pre-order, 2 post-order, 3 the mix rule.  Returns 0 or a negative
 // EITHOT_ERR_*.

@@ -41,15 +41,17 @@ void set_peak(size_t b) { g_peak = b; }
 
 Device* device_ctx(int* err) {
   static std::mutex mu;
-  static std::vector<std::unique_ptr<Device>> devs;
+  // never destroyed: device memory is not freed from static destructors at
+  // process exit, when the HIP runtime may already be gone
+  static auto* devs = new std::vector<std::unique_ptr<Device>>();
   int id = 0;
   if (hipGetDevice(&id) != hipSuccess) {
     if (err) *err = fail(EITHOT_ERR_NO_DEVICE, "no HIP device available");
     return nullptr;
   }
   std::lock_guard<std::mutex> lk(mu);
-  if ((int)devs.size() <= id) devs.resize(id + 1);
-  if (!devs[id]) {
+  if ((int)devs->size() <= id) devs->resize(id + 1);
+  if (!(*devs)[id]) {
     auto d = std::make_unique<Device>();
     d->id = id;
     hipDeviceProp_t prop;
@@ -71,9 +73,9 @@ Device* device_ctx(int* err) {
       (void)hipGetLastError();
     }
     d->cfg.lds_target = d->prof.lds_target;
-    devs[id] = std::move(d);
+    (*devs)[id] = std::move(d);
   }
-  return devs[id].get();
+  return (*devs)[id].get();
 }
 
 DeviceProfile Device::profile() const {
@@ -96,8 +98,8 @@ LaunchCfg Device::launch_cfg() const {
   return c;
 }
 
-const Plan* Device::plan_for(const int64_t* dims, const int* perm1, int rank, size_t elem,
-                             int n_sub, double alpha, int* rc) {
+PlanPtr Device::plan_for(const int64_t* dims, const int* perm1, int rank, size_t elem, int n_sub,
+                         double alpha, int* rc) {
   const DeviceProfile pr = profile();
   std::ostringstream k;
   k << tuning_get().epoch << "#";
@@ -106,31 +108,77 @@ const Plan* Device::plan_for(const int64_t* dims, const int* perm1, int rank, si
   for (int i = 0; i < rank; ++i) k << perm1[i] << ",";
   k << "|" << elem << "|" << n_sub << "|" << alpha;
   const std::string key = k.str();
-  std::lock_guard<std::mutex> lk(mu);
-  auto it = plans.find(key);
-  if (it != plans.end()) {
-    *rc = EITHOT_OK;
-    return it->second.get();
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    if (PlanPtr p = plans.get(key)) {
+      *rc = EITHOT_OK;
+      return p;
+    }
   }
-  auto p = std::make_unique<Plan>();
+  // plan without the lock (a large shape takes seconds); a plan another
+  // thread cached meanwhile wins
+  auto p = std::make_shared<Plan>();
   *rc = make_plan(dims, perm1, rank, elem, n_sub, alpha, pr, pr.reorder, *p);
   if (*rc != EITHOT_OK) return nullptr;
-  if (plans.size() > 4096) plans.clear();
-  const Plan* raw = p.get();
-  plans[key] = std::move(p);
-  return raw;
+  std::vector<PlanPtr> evicted;  // destroyed after the lock is released
+  PlanPtr out;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    if ((out = plans.get(key))) return out;
+    out = p;
+    evicted = plans.put(key, out, (uint64_t)std::max<int64_t>(p->table_bytes, 0));
+  }
+  return out;
 }
 
-int Device::reserve_arena(int64_t bytes, hipStream_t stream) {
-  if (bytes <= 0) return EITHOT_OK;
+int64_t Device::held_bytes() {
   std::lock_guard<std::mutex> lk(mu);
+  int64_t n = arena_bytes;
+  plans.for_each([&](const std::string&, const PlanPtr& p) { n += p->device_table_bytes(id); });
+  return n;
+}
+
+void Device::trim_locked(int64_t budget, const Plan* keep) {
+  std::vector<PlanPtr> order;  // most recently used first
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    plans.for_each([&](const std::string&, const PlanPtr& p) { order.push_back(p); });
+  }
+  int64_t keep_tables = keep ? std::max(keep->table_bytes, keep->device_table_bytes(id)) : 0;
+  int64_t others = 0;
+  for (const PlanPtr& p : order)
+    if (p.get() != keep) others += p->device_table_bytes(id);
+  const int64_t need = keep ? keep->scratch_bytes : 0;
+  if (arena_bytes + keep_tables + others <= budget) return;
+  // an arena larger than this call needs is given back first
+  if (arena && arena_bytes > need) {
+    if (arena_ready) (void)hipEventSynchronize(arena_ready);
+    (void)hipFree(arena);
+    arena = nullptr;
+    arena_bytes = 0;
+  }
+  // then the tables of other plans, least recently used first
+  for (auto it = order.rbegin(); it != order.rend(); ++it) {
+    if (arena_bytes + keep_tables + others <= budget) break;
+    if (it->get() == keep) continue;
+    const int64_t b = (*it)->device_table_bytes(id);
+    if (!b) continue;
+    (*it)->drop_device_tables(id);
+    others -= b;
+  }
+}
+
+int Device::reserve_arena_locked(int64_t bytes, hipStream_t stream) {
+  if (bytes <= 0) return EITHOT_OK;
   if (arena_ready) {  // earlier async users of the arena must finish first
     if (hipStreamWaitEvent(stream, arena_ready, 0) != hipSuccess)
       return fail(EITHOT_ERR_DEVICE, "hipStreamWaitEvent failed");
   }
   if (arena_bytes >= bytes) return EITHOT_OK;
   if (arena) {
-    hipDeviceSynchronize();
+    // every earlier user is done once its event has completed (users
+    // enqueue under the same lock)
+    if (arena_ready) (void)hipEventSynchronize(arena_ready);
     hipFree(arena);
     arena = nullptr;
     arena_bytes = 0;
@@ -144,8 +192,7 @@ int Device::reserve_arena(int64_t bytes, hipStream_t stream) {
   return EITHOT_OK;
 }
 
-int Device::release_arena_after(hipStream_t stream) {
-  std::lock_guard<std::mutex> lk(mu);
+int Device::release_arena_after_locked(hipStream_t stream) {
   if (!arena_ready && hipEventCreateWithFlags(&arena_ready, hipEventDisableTiming) != hipSuccess)
     return fail(EITHOT_ERR_DEVICE, "hipEventCreate failed");
   if (hipEventRecord(arena_ready, stream) != hipSuccess)
@@ -236,23 +283,32 @@ static int transpose_impl(void* dev, const int64_t* dims, const int* perm, int o
     (void)hipGetLastError();
     return fail(EITHOT_ERR_INVALID, "pointer is not device memory");
   }
-  const Plan* plan = d->plan_for(dims, perm, order, elem, n_sub, alpha, &rc);
+  PlanPtr plan = d->plan_for(dims, perm, order, elem, n_sub, alpha, &rc);
   if (!plan) {
     if (!g_err.size()) fail(rc, "planning failed (%d)", rc);
     return rc;
   }
-  if (plan->scratch_bytes > 0) {
-    rc = d->reserve_arena(plan->scratch_bytes, stream);
+  {
+    std::lock_guard<std::mutex> lk(d->enqueue_mu);
+    // the library's device bytes stay within alpha of this tensor (or this
+    // plan's own need, when one footprint exceeds it)
+    const int64_t own = plan->scratch_bytes + plan->table_bytes;
+    const int64_t budget =
+        std::max<int64_t>((int64_t)(alpha * (double)plan->tensor_bytes), own);
+    d->trim_locked(budget, plan.get());
+    if (plan->scratch_bytes > 0) {
+      rc = d->reserve_arena_locked(plan->scratch_bytes, stream);
+      if (rc) return rc;
+    }
+    std::vector<float>* times = g_timing ? &g_pass_ms : nullptr;
+    rc = run_plan(d, *plan, (char*)dev, d->arena, plan->scratch_bytes, stream, times);
     if (rc) return rc;
+    if (plan->scratch_bytes > 0) {
+      rc = d->release_arena_after_locked(stream);
+      if (rc) return rc;
+    }
   }
-  std::vector<float>* times = g_timing ? &g_pass_ms : nullptr;
-  rc = run_plan(d, *plan, (char*)dev, d->arena, plan->scratch_bytes, stream, times);
-  if (rc) return rc;
-  if (plan->scratch_bytes > 0) {
-    rc = d->release_arena_after(stream);
-    if (rc) return rc;
-  }
-  g_peak = (size_t)(plan->scratch_bytes + plan->table_bytes);
+  g_peak = (size_t)d->held_bytes();
   if (sync) {
     hipError_t e = hipStreamSynchronize(stream);
     if (e == hipSuccess) e = hipGetLastError();
@@ -301,12 +357,19 @@ void eithot_release(void) {
   int err = 0;
   Device* d = device_ctx(&err);
   if (!d) return;
-  std::lock_guard<std::mutex> lk(d->mu);
-  hipDeviceSynchronize();
-  if (d->arena) hipFree(d->arena);
+  std::vector<PlanPtr> dropped;
+  std::lock_guard<std::mutex> lk(d->enqueue_mu);
+  (void)hipDeviceSynchronize();
+  if (d->arena) (void)hipFree(d->arena);
   d->arena = nullptr;
   d->arena_bytes = 0;
-  d->plans.clear();
+  {
+    std::lock_guard<std::mutex> lk2(d->mu);
+    dropped = d->plans.clear();
+  }
+  // device tables are freed now; a plan still held elsewhere (a queued
+  // server request) uploads its tables again when it runs
+  for (const PlanPtr& p : dropped) p->drop_device_tables(d->id);
 }
 
 int eithot_estimate(const int64_t* dims, const int* perm, int order, size_t elem_bytes,
@@ -393,6 +456,21 @@ int eithot_plan_describe(const int64_t* dims, const int* perm, int order, size_t
     x.binv = o.binv;
   }
   return n;
+}
+
+int eithot_plan_partition(const int64_t* dims, const int* perm, int order, size_t elem_bytes,
+                          int n_sub, double alpha, int* n_sub_used, int* dim, int64_t* part_len,
+                          int64_t* last_len) {
+  DeviceProfile prof;
+  tuning_apply(tuning_get(), prof);
+  Plan plan;
+  int rc = make_plan(dims, perm, order, elem_bytes, n_sub, alpha, prof, prof.reorder, plan);
+  if (rc) return fail(rc, "planning failed (%d)", rc);
+  if (n_sub_used) *n_sub_used = plan.n_sub;
+  if (dim) *dim = plan.partition_dim;
+  if (part_len) *part_len = plan.part_len;
+  if (last_len) *last_len = plan.last_len;
+  return EITHOT_OK;
 }
 
 int64_t eithot_plan_block_table(const int64_t* dims, const int* perm, int order,

@@ -3,7 +3,6 @@
 
 #include <hip/hip_runtime.h>
 
-#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -11,26 +10,53 @@
 
 #include "kernels.hpp"
 #include "plan.hpp"
+#include "plan_cache.hpp"
 
 namespace eithot {
 
+using PlanPtr = std::shared_ptr<const Plan>;
+
+// Per-device state.  Two locks:
+//   mu         the plan cache;
+//   enqueue_mu held from the first use of the arena or of pass tables to the
+//              event recorded after a call's last pass: arena growth or
+//              shrinking, table eviction and every launch that uses them are
+//              mutually exclusive, so no thread launches into memory another
+//              thread frees.  Lock order: enqueue_mu, then mu.
 struct Device {
   int id = 0;
   DeviceProfile prof;
   LaunchCfg cfg;
   std::mutex mu;
-  std::map<std::string, std::unique_ptr<Plan>> plans;
-  char* arena = nullptr;       // scratch for staged passes, grown on demand
+  LruCache<Plan> plans{kPlanCacheEntries, kPlanCacheHostBytes};
+  std::mutex enqueue_mu;
+  char* arena = nullptr;             // scratch for staged passes and cycle boundaries
   int64_t arena_bytes = 0;
-  hipEvent_t arena_ready = nullptr;  // last stream use of the arena
+  hipEvent_t arena_ready = nullptr;  // recorded after the last pass that used the arena
 
-  const Plan* plan_for(const int64_t* dims, const int* perm1, int rank, size_t elem, int n_sub,
-                       double alpha, int* rc);
+  // plans cached per device, and the host bytes of their pass tables
+  static constexpr size_t kPlanCacheEntries = 1024;
+  static constexpr uint64_t kPlanCacheHostBytes = 16ull << 30;
+
+  // The cached plan for these arguments, planned on first sight (outside
+  // the cache lock).  The caller's reference keeps the plan alive even if
+  // the cache evicts it.
+  PlanPtr plan_for(const int64_t* dims, const int* perm1, int rank, size_t elem, int n_sub,
+                   double alpha, int* rc);
   // hardware profile / launch configuration with the tuning knobs applied
   DeviceProfile profile() const;
   LaunchCfg launch_cfg() const;
-  int reserve_arena(int64_t bytes, hipStream_t stream);
-  int release_arena_after(hipStream_t stream);
+  // With enqueue_mu held: keeps the device bytes the library holds (arena
+  // plus the resident tables of cached plans) within `budget` by freeing the
+  // tables of other cached plans, least recently used first, and shrinking
+  // an arena larger than `keep` needs.
+  void trim_locked(int64_t budget, const Plan* keep);
+  // With enqueue_mu held: arena of at least `bytes`, ordered after its last
+  // user on `stream`.
+  int reserve_arena_locked(int64_t bytes, hipStream_t stream);
+  int release_arena_after_locked(hipStream_t stream);
+  // Device bytes the library holds for plain transposes on this device.
+  int64_t held_bytes();
 };
 
 Device* device_ctx(int* err);

@@ -7,6 +7,11 @@
 // request consumes).  Admitted requests are enqueued on one of several
 // streams, so passes of different requests overlap on the device; a host
 // callback behind each request's last pass stamps its completion time.
+//
+// Requests on overlapping bytes (a tensor transposed and later transposed
+// back) run in submission order: admission is FIFO, and an admitted request
+// whose bytes overlap a running one joins that request's stream and waits on
+// the completion event of every other running request it overlaps.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -22,6 +27,7 @@
 #include "kernels.hpp"
 #include "plan.hpp"
 #include "runtime.hpp"
+#include "scratch_pool.hpp"
 
 using namespace eithot;
 
@@ -32,7 +38,9 @@ using Clock = std::chrono::steady_clock;
 struct Request {
   int64_t id = 0;
   char* dev = nullptr;
-  const Plan* plan = nullptr;
+  uint64_t nbytes = 0;   // the tensor occupies [dev, dev + nbytes)
+  PlanPtr plan;          // held until the request completes
+  hipEvent_t done_ev = nullptr;  // recorded after the request's last pass
   int status = 0;  // 0 queued, 1 running, 2 done, <0 error
   Clock::time_point t_submit;
   std::atomic<int64_t> done_ns{0};  // completion stamp (ns since t_submit), 0 = pending
@@ -42,46 +50,7 @@ struct Request {
   double latency_ms = 0;
 };
 
-// first-fit allocator over [0, cap)
-struct Pool {
-  int64_t cap = 0;
-  std::map<int64_t, int64_t> free_;  // offset -> length
-  void init(int64_t c) {
-    cap = c;
-    free_.clear();
-    if (c > 0) free_[0] = c;
-  }
-  int64_t alloc(int64_t n) {
-    if (n <= 0) return 0;
-    n = (n + 255) & ~(int64_t)255;
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second >= n) {
-        const int64_t off = it->first, len = it->second;
-        free_.erase(it);
-        if (len > n) free_[off + n] = len - n;
-        return off;
-      }
-    }
-    return -1;
-  }
-  void release(int64_t off, int64_t n) {
-    if (n <= 0 || off < 0) return;
-    n = (n + 255) & ~(int64_t)255;
-    auto it = free_.emplace(off, n).first;
-    auto nx = std::next(it);
-    if (nx != free_.end() && it->first + it->second == nx->first) {
-      it->second += nx->second;
-      free_.erase(nx);
-    }
-    if (it != free_.begin()) {
-      auto pv = std::prev(it);
-      if (pv->first + pv->second == it->first) {
-        pv->second += it->second;
-        free_.erase(it);
-      }
-    }
-  }
-};
+using Pool = ScratchPool;
 
 void on_done(void* p) {
   Request* r = static_cast<Request*>(p);
@@ -103,7 +72,8 @@ struct eithot_server {
   int64_t next_id = 0;
   std::map<int64_t, std::unique_ptr<Request>> reqs;
   std::deque<Request*> queue;
-  std::vector<Request*> running;
+  std::vector<Request*> running;  // in admission order
+  std::vector<hipEvent_t> free_events;
   double alpha = 0.05;
 
   ~eithot_server() {
@@ -111,7 +81,25 @@ struct eithot_server {
       (void)hipStreamSynchronize(s);
       (void)hipStreamDestroy(s);
     }
+    for (Request* r : running)
+      if (r->done_ev) free_events.push_back(r->done_ev);
+    for (hipEvent_t e : free_events) (void)hipEventDestroy(e);
     if (pool_mem) (void)hipFree(pool_mem);
+  }
+
+  hipEvent_t take_event() {
+    if (!free_events.empty()) {
+      hipEvent_t e = free_events.back();
+      free_events.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    return e;
+  }
+
+  static bool overlaps(const Request* a, const Request* b) {
+    return a->dev < b->dev + b->nbytes && b->dev < a->dev + a->nbytes;
   }
 
   // retire finished requests; returns how many completed
@@ -128,8 +116,10 @@ struct eithot_server {
       r->status = 2;
       pool.release(r->off, r->bytes);
       --stream_load[r->stream];
-      running[i] = running.back();
-      running.pop_back();
+      free_events.push_back(r->done_ev);
+      r->done_ev = nullptr;
+      r->plan.reset();
+      running.erase(running.begin() + (long)i);  // keeps admission order
       ++n;
     }
     return n;
@@ -146,18 +136,43 @@ struct eithot_server {
       }
       r->off = off;
       r->bytes = need;
+      r->done_ev = take_event();
+      if (!r->done_ev) {
+        pool.release(off, need);
+        break;  // retried on the next poll
+      }
+      // the least-loaded stream, unless the request's bytes overlap a
+      // running request: then the stream of the last such request
       int s = 0;
       for (int k = 1; k < (int)streams.size(); ++k)
         if (stream_load[k] < stream_load[s]) s = k;
+      for (const Request* o : running)
+        if (overlaps(r, o)) s = o->stream;
       r->stream = s;
       ++stream_load[s];
       queue.pop_front();
       hipStream_t st = streams[s];
-      int rc = run_plan(dev, *r->plan, r->dev, need > 0 ? pool_mem + off : nullptr, need, st, nullptr);
+      int rc = EITHOT_OK;
+      for (const Request* o : running)
+        if (overlaps(r, o) && o->stream != s && hipStreamWaitEvent(st, o->done_ev, 0) != hipSuccess)
+          rc = EITHOT_ERR_DEVICE;
+      if (rc == EITHOT_OK) {
+        // launches are exclusive with other threads' table eviction
+        std::lock_guard<std::mutex> lk(dev->enqueue_mu);
+        rc = run_plan(dev, *r->plan, r->dev, need > 0 ? pool_mem + off : nullptr, need, st,
+                      nullptr);
+      }
+      if (rc == EITHOT_OK && hipEventRecord(r->done_ev, st) != hipSuccess) rc = EITHOT_ERR_DEVICE;
       if (rc == EITHOT_OK && hipLaunchHostFunc(st, on_done, r) != hipSuccess) rc = EITHOT_ERR_DEVICE;
       if (rc != EITHOT_OK) {
+        // passes already enqueued may still run: wait for them before the
+        // scratch and the event are reused
+        (void)hipStreamSynchronize(st);
         r->status = rc;
         pool.release(r->off, r->bytes);
+        free_events.push_back(r->done_ev);
+        r->done_ev = nullptr;
+        r->plan.reset();
         --stream_load[s];
         continue;
       }
@@ -201,16 +216,17 @@ int64_t eithot_server_submit(eithot_server* s, void* dev, const int64_t* dims, c
   if (!s) return fail(EITHOT_ERR_INVALID, "null server");
   int rc = validate(dev, dims, perm, order, elem_bytes);
   if (rc) return rc;
-  const Plan* plan = s->dev->plan_for(dims, perm, order, elem_bytes, 1, s->alpha, &rc);
+  PlanPtr plan = s->dev->plan_for(dims, perm, order, elem_bytes, 1, s->alpha, &rc);
   if (!plan) return rc ? rc : fail(EITHOT_ERR_INVALID, "planning failed");
-  if (plan->scratch_bytes > s->pool.cap)
+  if (!s->pool.fits(plan->scratch_bytes))
     return fail(EITHOT_ERR_ALPHA, "request needs %lld scratch bytes, pool holds %lld",
-                (long long)plan->scratch_bytes, (long long)s->pool.cap);
+                (long long)Pool::granule(plan->scratch_bytes), (long long)s->pool.cap);
   std::lock_guard<std::mutex> lk(s->mu);
   auto r = std::make_unique<Request>();
   r->id = s->next_id++;
   r->dev = (char*)dev;
-  r->plan = plan;
+  r->nbytes = plan->tensor_bytes;
+  r->plan = std::move(plan);
   r->t_submit = Clock::now();
   Request* raw = r.get();
   s->reqs[raw->id] = std::move(r);
@@ -252,6 +268,10 @@ int eithot_server_drain(eithot_server* s) {
       s->retire();
       s->admit();
       if (s->queue.empty() && s->running.empty()) return EITHOT_OK;
+      // nothing runs and the head still cannot start: it never will
+      if (s->running.empty())
+        return fail(EITHOT_ERR_DEVICE, "request %lld cannot be admitted",
+                    (long long)s->queue.front()->id);
     }
     int rc = eithot_server_poll(s, 1);
     if (rc < 0) return rc;

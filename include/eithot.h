@@ -40,18 +40,24 @@ extern "C" {
 
 /* Transposes the tensor at device pointer `dev` in place and returns when it
  * is done.  n_sub > 1 asks for the reference's sub-tensor partitioning along
- * the largest dim (applied when that dim exceeds floor(32/alpha); the count
- * used is the largest divisor of the dim not above n_sub); n_sub <= 0 takes
- * the estimator's choice (eithot_estimate: 1, since this build's plans meet
- * alpha without partitioning).  Extra device memory stays <= alpha * tensor
- * bytes, or one pass footprint when that is larger.
+ * the largest dim, applied when that dim exceeds floor(32/alpha): the dim is
+ * cut into min(n_sub, dim) sub-tensors, the last one ragged when the count
+ * does not divide the dim (the reference pads the dim up to a multiple of
+ * n_sub instead, growing the tensor; here the buffer is exactly the tensor,
+ * with no headroom).  n_sub <= 0 takes the estimator's choice
+ * (eithot_estimate: 1, since this build's plans meet alpha without
+ * partitioning).  Extra device memory stays <= alpha * tensor bytes, or one
+ * pass footprint when that is larger.
  * Replaces inplace::transpose (ITNT/include/transpose.h:8,
  * ITNT/src/transpose.cpp:21-30). */
 int eithot_transpose(void* dev, const int64_t* dims, const int* perm, int order,
                      size_t elem_bytes, int n_sub, double alpha);
 
 /* Same, enqueued on `stream` without a host synchronisation.  The scratch
- * arena stays reserved until the work completes; do not free `dev` first. */
+ * arena stays reserved until the work completes; do not free `dev` first.
+ * Calls may come from several threads: the launches of calls that use the
+ * arena or pass tables are enqueued one call at a time, and a call's first
+ * pass waits (on the device) for the last pass of the previous arena user. */
 int eithot_transpose_async(void* dev, const int64_t* dims, const int* perm, int order,
                            size_t elem_bytes, int n_sub, double alpha, void* stream);
 
@@ -63,15 +69,22 @@ int eithot_inplace_transpose(void* data, int source, int rank, const int* dim,
                              const int* permutation, size_t sizeofType, int num_block,
                              double large_ratio);
 
-/* Peak extra device bytes used by the calling thread's last transpose:
- * scratch plus the tables of its cycle-following and fused passes; 0 when
- * every pass ran through LDS without tables. */
+/* Extra device bytes the library held during the calling thread's last
+ * transpose, measured rather than planned: its scratch arena as allocated
+ * plus every pass table resident on the device for the plans it caches (this
+ * call's and earlier ones').  Each call first frees other plans' tables,
+ * least recently used first, and shrinks an arena larger than it needs, so
+ * that this stays within alpha * tensor bytes, or within the call's own
+ * need when one pass footprint exceeds that.  0 when nothing is held.
+ * Server pools and batch scratch belong to their handles and are not
+ * counted. */
 size_t eithot_last_peak_extra_bytes(void);
 
 /* Message of the calling thread's last failure ("" if none). */
 const char* eithot_last_error(void);
 
-/* Frees the scratch arenas and cached plans of every device. */
+/* Frees the scratch arena, the cached plans and their device tables of the
+ * current device, after waiting for its work to finish. */
 void eithot_release(void);
 
 /* ---- planner / estimator (host only, no GPU needed) ------------------------ */
@@ -128,6 +141,15 @@ typedef struct {
 int eithot_plan_describe(const int64_t* dims, const int* perm, int order, size_t elem_bytes,
                          int n_sub, double alpha, int64_t lds_max, eithot_op_desc* ops,
                          int max_ops, int64_t* scratch_bytes);
+
+/* Sub-tensor partition of the plan: *n_sub_used sub-tensors along 0-based
+ * input dim *dim (-1 and 1 when the plan does not partition), the first
+ * n_sub_used - 1 holding *part_len indices of that dim and the last
+ * *last_len.  Returns 0 or a negative EITHOT_ERR_*.  The reference prints
+ * the count as "number of sub_tensors" (ITNT/src/test_inplace.cpp:34). */
+int eithot_plan_partition(const int64_t* dims, const int* perm, int order, size_t elem_bytes,
+                          int n_sub, double alpha, int* n_sub_used, int* dim, int64_t* part_len,
+                          int64_t* last_len);
 
 /* Gather table of pass op_index when it is a fused block permutation:
  * output cell o of every block takes input cell src[o].  Returns the cell
@@ -188,6 +210,10 @@ int eithot_plan_segments(const int64_t* dims, const int* perm, int order, size_t
  *   "lds_threads_auto" tile, column, row and block workgroups sized for one round of
  *                    8 loads per thread (1) or from "threads"/"threads_big" (0)
  *   "xcd_remap"      keep consecutive footprints on one XCD (1)
+ *   "lower_mask"     lowering options of each 2-D transpose (31 = all): 1 LDS
+ *                    tile, 2 Catanzaro column/row passes, 4 hierarchical split,
+ *                    8 tile through scratch, 16 cycle following; a transpose
+ *                    no allowed option can lower fails with EITHOT_ERR_ALPHA
  * Returns EITHOT_ERR_INVALID for an unknown key or an out-of-range value. */
 int eithot_set_tuning(const char* key, int64_t value);
 int eithot_get_tuning(const char* key, int64_t* value);
@@ -232,10 +258,15 @@ void eithot_batch_destroy(eithot_batch* b);
 typedef struct eithot_server eithot_server;
 
 /* scratch_pool_bytes is the shared extra-memory budget all in-flight requests
- * are admitted against; max_inflight caps concurrent requests; nstreams is
- * the number of device streams requests are spread over. */
+ * are admitted against (used in whole 256-byte granules); max_inflight caps
+ * concurrent requests; nstreams is the number of device streams requests are
+ * spread over. */
 eithot_server* eithot_server_create(int64_t scratch_pool_bytes, int max_inflight, int nstreams);
-/* Enqueues a request on resident tensor `dev`; returns a request id >= 0. */
+/* Enqueues a request on resident tensor `dev`; returns a request id >= 0.
+ * Requests are admitted in submission order.  Requests whose bytes overlap
+ * run one after the other in that order (a tensor may be transposed and then
+ * transposed back); others may run concurrently.  A request whose scratch,
+ * rounded up to 256 bytes, exceeds the pool is refused (EITHOT_ERR_ALPHA). */
 int64_t eithot_server_submit(eithot_server* s, void* dev, const int64_t* dims, const int* perm,
                              int order, size_t elem_bytes);
 /* Admits queued requests while slots and scratch allow and retires finished
@@ -245,7 +276,8 @@ int eithot_server_poll(eithot_server* s, int wait);
 /* Status of a request: 0 queued, 1 running, 2 done, <0 error code.
  * On done, *latency_ms holds submit-to-completion time (host clock). */
 int eithot_server_status(eithot_server* s, int64_t id, double* latency_ms);
-/* Blocks until every submitted request has completed. */
+/* Blocks until every submitted request has completed; returns an error
+ * instead of waiting forever if a queued request can never be admitted. */
 int eithot_server_drain(eithot_server* s);
 void eithot_server_destroy(eithot_server* s);
 

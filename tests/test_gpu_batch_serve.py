@@ -125,3 +125,47 @@ def test_server_refuses_requests_it_cannot_admit(torch_cuda):
     with pytest.raises(E.EithotError):
         srv.submit(t.data_ptr(), [3, 200003], [2, 1], 8)
     srv.close()
+
+
+def test_server_orders_requests_on_one_tensor(torch_cuda):
+    # forward, then inverse, then forward again on one resident tensor, with
+    # unrelated requests between them on a 4-stream server: the requests on
+    # the shared bytes run in submission order
+    import eithot_amd as E
+    torch = torch_cuda
+    dims, perm, elem = [32, 48, 16, 40, 9, 8], [6, 4, 2, 5, 1, 3], 4
+    inv_dims = [dims[p - 1] for p in perm]
+    inv = [0] * len(perm)
+    for k, p in enumerate(perm):
+        inv[p - 1] = k + 1
+    vol = int(np.prod(dims))
+    host = O.fill(vol, elem, 31, 1)
+    t = torch.from_numpy(host.copy()).to("cuda:0")
+    rng = np.random.default_rng(32)
+    others = [random_tensor(rng, orders=(3, 5), dmax=100, vmax=500_000) for _ in range(6)]
+    obuf, ooffs = carve(torch, others)
+    ohost = []
+    for k, ((d, _, e), off) in enumerate(zip(others, ooffs)):
+        h = O.fill(int(np.prod(d)), e, 600 + k, 1)
+        ohost.append(h)
+        obuf[off:off + h.size] = torch.from_numpy(h).to("cuda:0")
+    torch.cuda.synchronize()
+    srv = E.Server(scratch_pool_bytes=64 << 20, max_inflight=8, nstreams=4)
+    p = t.data_ptr()
+    seq = [(p, dims, perm), (p, inv_dims, inv), (p, dims, perm)]
+    ids = []
+    for k in range(len(seq)):
+        ptr, d, q = seq[k]
+        ids.append(srv.submit(ptr, d, q, elem))
+        for j in range(2 * k, 2 * k + 2):
+            d2, q2, e2 = others[j]
+            srv.submit(obuf.data_ptr() + ooffs[j], d2, q2, e2)
+    srv.drain()
+    for rid in ids:
+        assert srv.status(rid)[0] == 2
+    srv.close()
+    want = O.permute(host, dims, [q - 1 for q in perm], elem)
+    assert_bitexact(t.cpu().numpy(), want, "forward, inverse, forward on one tensor")
+    for k, ((d, q, e), off, h) in enumerate(zip(others, ooffs, ohost)):
+        assert_bitexact(obuf[off:off + h.size].cpu().numpy(),
+                        O.permute(h, d, [x - 1 for x in q], e), f"other request {k}")

@@ -24,15 +24,45 @@ def test_all_perms_rank4_small(torch_cuda, elem):
         assert_bitexact(got, want, f"perm {perm}")
 
 
-def test_catanzaro_lds_shapes(torch_cuda):
-    # column/row passes on LDS: gcd > 1 (rotate present) and gcd == 1, C2R and R2C
-    for dims in ([384, 256], [256, 384], [1000, 999], [999, 1000], [2, 640, 480, 2],
-                 [4, 96, 72, 3]):
-        r = len(dims)
-        perm = [2, 1] if r == 2 else [1, 3, 2, 4]
-        for elem in (1, 4, 8):
-            got, want, _ = run_case(torch_cuda, dims, perm, elem, seed=3)
-            assert_bitexact(got, want, f"{dims} elem {elem}")
+# Catanzaro passes (kind, map, staged) expected when only that lowering is
+# allowed: C2R = [rotate] -> row gather -> column shuffle, R2C = column
+# shuffle inverse -> row scatter (gather form) -> [rotate inverse]
+ROT, ROTINV, COLSH, COLSHINV, ROWG, ROWD = 1, 2, 3, 4, 5, 6
+CATANZARO_CASES = [
+    # gcd > 1 (rotate passes), columns and rows in LDS
+    ([384, 256], [2, 1], 4, 0.05, {(2, ROT, 0), (3, ROWG, 0), (2, COLSH, 0)}),
+    ([256, 384], [2, 1], 4, 0.05, {(2, COLSHINV, 0), (3, ROWD, 0), (2, ROTINV, 0)}),
+    ([2, 640, 480, 2], [1, 3, 2, 4], 4, 0.05, {(2, ROT, 0), (3, ROWG, 0), (2, COLSH, 0)}),
+    ([1024, 4098], [2, 1], 4, 0.05, {(2, COLSHINV, 0), (3, ROWD, 0), (2, ROTINV, 0)}),
+    # gcd == 1: no rotate
+    ([1000, 999], [2, 1], 1, 0.05, {(3, ROWG, 0), (2, COLSH, 0)}),
+    ([999, 1000], [2, 1], 8, 0.05, {(2, COLSHINV, 0), (3, ROWD, 0)}),
+    # columns of 3000 one-byte units exceed LDS: staged column passes,
+    # rotate included
+    ([3002, 3000], [2, 1], 1, 1.0, {(2, ROT, 1), (3, ROWG, 0), (2, COLSH, 1)}),
+    ([3000, 3002], [2, 1], 1, 1.0, {(2, COLSHINV, 1), (3, ROWD, 0), (2, ROTINV, 1)}),
+    # rows of 100003 units exceed LDS: staged row gather / scatter
+    ([100003, 3], [2, 1], 4, 1.0, {(3, ROWG, 1), (2, COLSH, 0)}),
+    ([3, 100003], [2, 1], 4, 1.0, {(2, COLSHINV, 0), (3, ROWD, 1)}),
+]
+
+
+@pytest.mark.parametrize("dims,perm,elem,alpha,want_ops", CATANZARO_CASES)
+def test_catanzaro_passes(torch_cuda, dims, perm, elem, alpha, want_ops):
+    # only the Catanzaro lowering allowed (no tile, split, cycle or regroup),
+    # so the column / row kernels run with every map, in LDS and staged
+    import eithot_amd as E
+    old = {k: E.set_tuning(k, v) for k, v in (("lower_mask", 2), ("regroup", 0))}
+    try:
+        ops, _ = E.plan_describe(dims, perm, elem, 1, alpha)
+        got_ops = {(o["kind"], o["map"], o["staged"]) for o in ops}
+        assert got_ops == want_ops, (dims, got_ops)
+        got, want, peak = run_case(torch_cuda, dims, perm, elem, seed=3, alpha=alpha)
+        assert_bitexact(got, want, f"catanzaro {dims} {perm} elem {elem}")
+        assert peak <= max(alpha * got.nbytes, 1)
+    finally:
+        for k, v in old.items():
+            E.set_tuning(k, v)
 
 
 def test_staged_passes(torch_cuda):

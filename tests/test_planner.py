@@ -234,6 +234,52 @@ def test_partition_plan_executes_correctly():
         assert np.array_equal(b, O.permute(a, dims, [p - 1 for p in perm1], 4))
 
 
+RAGGED_CASES = [([3, 100003], [2, 1], 4, 4), ([100003, 3], [2, 1], 4, 4),
+                ([7, 1009, 5], [3, 1, 2], 2, 3), ([7, 1009, 5], [2, 3, 1], 1, 7),
+                ([5, 6, 331, 4], [4, 3, 1, 2], 8, 5), ([331, 2, 3], [3, 1, 2], 4, 64),
+                ([2, 4, 67, 3], [1, 3, 2, 4], 4, 10), ([90, 45], [2, 1], 8, 4),
+                ([41], [1], 4, 3), ([2, 163, 5, 2], [1, 3, 2, 4], 4, 7)]
+
+
+@pytest.mark.parametrize("dims,perm1,elem,nsub", RAGGED_CASES)
+def test_ragged_partition_plans(dims, perm1, elem, nsub):
+    # the requested sub-tensor count holds for split dims it does not divide:
+    # nsub - 1 equal sub-tensors and a ragged last one; the plan run on the
+    # host equals the oracle's permutation, and needs no padding headroom
+    vol = int(np.prod(dims))
+    n, d, q, r = E.plan_partition(dims, perm1, elem, nsub, 1.0)
+    big = int(np.argmax(dims))
+    if dims[big] <= 32 or perm1 == list(range(1, len(dims) + 1)):
+        assert n == 1
+        return
+    assert n == min(nsub, dims[big]) and d == big
+    assert (n - 1) * q + r == dims[big] and q >= 1 and r >= 1
+    ops, _ = E.plan_describe(dims, perm1, elem, nsub, 1.0)
+    assert any(o["stage"] == -1 for o in ops)
+    assert max(o["base"] + o["B"] * o["R"] * o["C"] * o["U"] for o in ops) <= vol * elem
+    a = O.fill(vol, elem, 17, 1)
+    b = a.copy()
+    simulate(b, ops)
+    assert np.array_equal(b, O.permute(a, dims, [p - 1 for p in perm1], elem))
+
+
+def test_ragged_partition_random():
+    rng = np.random.default_rng(31)
+    for _ in range(150):
+        r = int(rng.integers(1, 6))
+        dims = [int(x) for x in rng.integers(1, 9, size=r)]
+        dims[int(rng.integers(r))] = int(rng.integers(33, 400))
+        perm = [int(p) + 1 for p in rng.permutation(r)]
+        elem = int(rng.choice([1, 2, 4, 8]))
+        nsub = int(rng.integers(2, 70))
+        vol = int(np.prod(dims))
+        ops, _ = E.plan_describe(dims, perm, elem, nsub, 1.0)
+        a = O.fill(vol, elem, 18, 1)
+        b = a.copy()
+        simulate(b, ops)
+        assert np.array_equal(b, O.permute(a, dims, [p - 1 for p in perm], elem)), (dims, perm, nsub)
+
+
 def test_benchmark_shape_plans():
     # (2,4096,1024,2) perm 1 3 2 4: at most the three Catanzaro passes of the reference's lowering,
     # scratch within 5 % of 64 MiB
@@ -347,3 +393,37 @@ def test_lds_footprints_fit_the_workgroup_limit(lds):
             else:
                 need = o["C"] * o["us"]
             assert need <= limit, (dims, perm1, elem, o)
+
+
+def test_catanzaro_forced_plans():
+    # the GPU suite's Catanzaro cases plan the passes it asserts, and those
+    # plans are correct when run on the host
+    from test_gpu_parity import CATANZARO_CASES
+    old = {k: E.set_tuning(k, v) for k, v in (("lower_mask", 2), ("regroup", 0))}
+    try:
+        for dims, perm, elem, alpha, want_ops in CATANZARO_CASES:
+            ops, _ = E.plan_describe(dims, perm, elem, 1, alpha)
+            assert {(o["kind"], o["map"], o["staged"]) for o in ops} == want_ops, dims
+            vol = int(np.prod(dims))
+            a = O.fill(vol, elem, 3, 1)
+            b = a.copy()
+            simulate(b, ops)
+            assert np.array_equal(b, O.permute(a, dims, [p - 1 for p in perm], elem)), dims
+    finally:
+        for k, v in old.items():
+            E.set_tuning(k, v)
+
+
+def test_lower_mask_knob():
+    # a mask that leaves no way to lower a transpose fails planning; an
+    # out-of-range mask is refused
+    old = E.set_tuning("lower_mask", 1)  # LDS tile only
+    try:
+        with pytest.raises(E.EithotError):
+            E.plan_describe([100003, 3], [2, 1], 4, 1, 0.05)
+        assert E.plan_describe([64, 32], [2, 1], 4, 1, 0.05)[0][0]["kind"] == 1
+    finally:
+        E.set_tuning("lower_mask", old)
+    for bad in (0, 32, -1):
+        with pytest.raises(E.EithotError):
+            E.set_tuning("lower_mask", bad)
