@@ -1,19 +1,28 @@
-"""Benchmark: in-place transposition of the 6-D high-order workload on one MI355X.
+"""Benchmark: in-place transposition on one MI355X.
 
-Workload: dims (32,48,64,40,36,8) [fastest first], uint32 elements (4.53 GB),
-random bits from seed 1, permutation (6 4 2 5 1 3), alpha 0.05.  A step is one
-in-place transpose of the whole tensor; steps alternate the permutation and
-its inverse so the tensor returns to its initial layout every two steps (both
-directions are timed).  The tensor is resident in HBM before timing starts.
+Headline workload: the 4-D tensor dims (2,163840,65536,2) [fastest first],
+uint32 elements (171.8 GB, more than half of the 288 GB HBM, so no
+out-of-place copy fits beside it), random bits from seed 2, permutation
+(1 3 2 4), alpha 0.05 with the sub-tensor count from the estimator.  A step is
+one in-place transpose of the whole tensor; steps alternate the permutation
+and its inverse so the tensor returns to its initial layout every two steps
+(both directions are timed).  The tensor is resident in HBM before timing
+starts.  The first forward result is checked on the device against the
+permuted pattern (and a sample against the oracle's closed form) before the
+timed steps, and the final state after them.
 
-Prints one JSON line: effective bandwidth (2 x tensor bytes / step time),
-the roofline of the dominant kernel (per-pass HIP events on the launching
-stream), and the CPU baseline (multithreaded out-of-place permute of the
-oracle, timed on this host).
+Prints one JSON line: effective bandwidth (2 x tensor bytes / step time), the
+roofline of the dominant kernel (per-pass HIP events on the launching
+stream), the 6-D high-order workload (dims (32,48,64,40,36,8), perm
+(6 4 2 5 1 3), 4.53 GB) as a secondary block, and the CPU baseline (the
+oracle's multithreaded out-of-place permute on a labelled scaled proxy,
+timed on this host).
 """
 import argparse
+import ctypes
 import json
 import os
+import platform
 import sys
 import time
 
@@ -21,11 +30,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-DIMS = [32, 48, 64, 40, 36, 8]
-PERM = [6, 4, 2, 5, 1, 3]
-ELEM = 4
-ALPHA = 0.05
-SEED = 1
+METRIC = "in-place transpose effective GB/s & % HBM peak; served tensors/s + p99 latency"
+C3 = {"name": "4-D beyond-half-HBM in-place transpose dims (2,163840,65536,2) perm (1 3 2 4), "
+              "4-byte elements, 171.8 GB, random uint32 from seed 2; steps alternate perm and "
+              "inverse",
+      "dims": [2, 163840, 65536, 2], "perm": [1, 3, 2, 4], "elem": 4, "alpha": 0.05, "seed": 2}
+C2 = {"name": "6-D high-order in-place transpose dims (32,48,64,40,36,8) perm (6 4 2 5 1 3), "
+              "4-byte elements, 4.53 GB, random uint32 from seed 1; steps alternate perm and "
+              "inverse",
+      "dims": [32, 48, 64, 40, 36, 8], "perm": [6, 4, 2, 5, 1, 3], "elem": 4, "alpha": 0.05,
+      "seed": 1}
+# HBM read+write traffic per launch of each kernel class on the headline
+# workload, from the FETCH_SIZE / WRITE_SIZE summary committed beside it
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_4d_172GB.json")
 
 
 def inverse(dims, perm1):
@@ -53,33 +70,98 @@ def kernel_name(op):
     return f"{kinds[op['kind']]}{maps[op['map']]}{'_staged' if op['staged'] else ''}"
 
 
-def cpu_baseline(threads):
-    """Oracle's multithreaded cache-tiled out-of-place permute on the same workload."""
+def host_info():
+    """Cores this process may use, the CPU model and the host RAM."""
+    cpus = len(os.sched_getaffinity(0))
+    model = platform.processor() or ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    ram = 0
+    try:
+        for line in open("/proc/meminfo"):
+            if line.startswith("MemTotal"):
+                ram = int(line.split()[1]) * 1024
+                break
+    except OSError:
+        pass
+    # worker threads: the cores this process may run on, within the host's
+    # per-job thread setting when one is given
+    threads = cpus
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        threads = min(threads, int(env))
+    return {"nproc": os.cpu_count(), "affinity_cpus": cpus, "threads": threads,
+            "cpu_model": model, "ram_bytes": ram}
+
+
+def _median_runs(fn, runs=5):
+    fn()  # warm-up
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2], ts
+
+
+def cpu_baseline():
+    """The oracle's multithreaded cache-tiled out-of-place permute, timed on
+    this host on a scaled proxy of the headline workload (the full 171.8 GB
+    tensor cannot be permuted out of place on the host: it needs twice its
+    size), plus the single-thread permute (the reference's seq_tt) on the
+    64 MB 4-D workload for context."""
     import numpy as np
 
     from oracle import pyoracle as O
-    vol = int(np.prod(DIMS))
-    src = np.empty(vol * ELEM, dtype=np.uint8)
-    # fill in chunks with the same generator (values do not affect timing)
+    info = host_info()
+    dims, perm0, elem = [2, 16384, 65536, 2], [0, 2, 1, 3], 4
+    vol = int(np.prod(dims))
+    src = np.empty(vol * elem, dtype=np.uint8)
     chunk = 1 << 26
     for off in range(0, vol, chunk):
         n = min(chunk, vol - off)
-        src[off * ELEM:(off + n) * ELEM] = O.fill(n, ELEM, SEED, 1, first=off)
+        O.lib().or_fill(O._ptr(src[off * elem:]), n, elem, C3["seed"], 1, off)
     dst = np.empty_like(src)
-    perm0 = [p - 1 for p in PERM]
     lib = O.lib()
-    times = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        rc = lib.or_permute_mt(O._ptr(src), O._ptr(dst), O._i64(DIMS), O._int(perm0), len(DIMS),
-                               ELEM, threads)
-        times.append(time.perf_counter() - t0)
+
+    def run_mt():
+        rc = lib.or_permute_mt(O._ptr(src), O._ptr(dst), O._i64(dims), O._int(perm0), len(dims),
+                               elem, info["threads"])
         assert rc == 0
-    t = sorted(times)[1]
-    return {"value": round(2 * vol * ELEM / t / 1e9, 3), "unit": "GB/s", "cores": threads,
-            "kind": "port",
-            "sample": "full workload (4.53 GB in, 4.53 GB out), median of 3 runs, "
-                      "oracle or_permute_mt (32x32-tiled, pthreads)"}
+
+    t_mt, ts = _median_runs(run_mt)
+    # spot check of the proxy's result against the closed form
+    for i in (0, 12345, vol - 1):
+        d = O.dest_index(i, dims, perm0)
+        assert int(dst[d * elem:(d + 1) * elem].view(np.uint32)[0]) == \
+            (O.value(C3["seed"], i, 1) & 0xFFFFFFFF)
+    del src, dst
+    d1, p1 = [2, 4096, 1024, 2], [0, 2, 1, 3]
+    v1 = int(np.prod(d1))
+    a = np.arange(v1, dtype=np.uint32).view(np.uint8)
+    b = np.empty_like(a)
+
+    def run_st():
+        assert lib.or_permute(O._ptr(a), O._ptr(b), O._i64(d1), O._int(p1), 4, 4) == 0
+
+    t_st, _ = _median_runs(run_st)
+    return {"value": round(2 * vol * elem / t_mt / 1e9, 3), "unit": "GB/s",
+            "cores": info["threads"], "kind": "port",
+            "sample": "scaled proxy of the headline workload: dims (2,16384,65536,2) perm "
+                      "(1 3 2 4), uint32, 8.59 GB in + 8.59 GB out (the 171.8 GB tensor cannot be "
+                      "permuted out of place on the host); median of 5 runs after 1 warm-up; "
+                      "oracle or_permute_mt (32x32-tiled, pthreads)",
+            "runs_s": [round(t, 4) for t in ts],
+            "host": info,
+            "single_thread_64MB": {"value": round(2 * v1 * 4 / t_st / 1e9, 3), "unit": "GB/s",
+                                   "workload": "dims (2,4096,1024,2) perm (1 3 2 4), uint32 "
+                                               "v(i)=i, oracle or_permute (one thread, the "
+                                               "reference's seq_tt)"}}
 
 
 def _carve(torch, specs, align=256):
@@ -348,31 +430,136 @@ def run_copy(nbytes=1 << 30, reps=5):
             "GBps_read_plus_write": round(2 * nbytes / t / 1e9, 1)}
 
 
+def _sampled_forward_check(buf, dims, perm, elem, seed, count=4000):
+    """Oracle closed form (destination index, generator value) of sampled
+    input elements against the device bytes; True when all agree."""
+    import numpy as np
+    import torch
+
+    from oracle import pyoracle as O
+    vol = _vol(dims)
+    rng = np.random.default_rng(seed + 7)
+    src = rng.integers(0, vol, size=count, dtype=np.int64)
+    dst = np.array([O.dest_index(int(i), dims, [q - 1 for q in perm]) for i in src], dtype=np.int64)
+    words = {1: torch.uint8, 2: torch.int16, 4: torch.int32, 8: torch.int64}[elem]
+    npw = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[elem]
+    got = buf.view(words)[torch.from_numpy(dst).to(buf.device)].cpu().numpy().view(npw)
+    mask = (1 << (8 * elem)) - 1
+    want = np.array([O.value(seed, int(i), 1) & mask for i in src], dtype=np.uint64)
+    return bool(np.array_equal(got.astype(np.uint64), want))
+
+
+def measure(spec, steps, warmup, label):
+    """Times `steps` in-place transposes of one resident tensor (alternating
+    the permutation and its inverse) after `warmup` untimed ones; per-pass
+    HIP events give each kernel class's duration.  Returns the result dict;
+    exits non-zero if a check fails."""
+    import torch
+
+    import eithot_amd as E
+    dims, perm, elem, alpha, seed = spec["dims"], spec["perm"], spec["elem"], spec["alpha"], \
+        spec["seed"]
+    vol = _vol(dims)
+    nbytes = vol * elem
+    torch.cuda.empty_cache()
+    buf = torch.empty(nbytes, dtype=torch.uint8, device="cuda:0")
+    E.fill_pattern(buf, vol, elem, seed, 1)
+    torch.cuda.synchronize()
+    n_sub, _, _ = E.estimate(dims, perm, elem, alpha)
+    inv_dims, inv_perm = inverse(dims, perm)
+    cases = [(dims, perm), (inv_dims, inv_perm)]
+    ident = list(range(1, len(dims) + 1))
+
+    def step(i):
+        d, p = cases[i % 2]
+        E.transpose_tensor(buf, d, p, elem, n_sub, alpha)
+
+    # first forward: plans the shape, then the result is checked
+    step(0)
+    torch.cuda.synchronize()
+    bad = E.verify_pattern(buf, dims, perm, elem, seed, 1)
+    if bad or not _sampled_forward_check(buf, dims, perm, elem, seed):
+        print(f"bench {label}: forward result differs from the permuted pattern ({bad} "
+              f"elements)", file=sys.stderr)
+        sys.exit(1)
+    step(1)
+    for i in range(warmup):
+        step(i)
+    if warmup % 2:
+        step(1)
+    torch.cuda.synchronize()
+    print(f"bench {label}: forward checked, warmup done ({warmup} steps)", file=sys.stderr,
+          flush=True)
+    E.lib().eithot_set_pass_timing(1)
+    pass_ms = {0: [], 1: []}
+    peak = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step(i)
+        n = E.lib().eithot_last_pass_times(None, 0)
+        arr = (ctypes.c_float * n)()
+        E.lib().eithot_last_pass_times(arr, n)
+        pass_ms[i % 2].append(list(arr))
+        peak = max(peak, E.last_peak_extra_bytes())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    E.lib().eithot_set_pass_timing(0)
+    print(f"bench {label}: timed {steps} steps, {elapsed * 1e3 / steps:.3f} ms/step",
+          file=sys.stderr, flush=True)
+    final = perm if steps % 2 else ident
+    bad = E.verify_pattern(buf, dims, final, elem, seed, 1)
+    if bad:
+        print(f"bench {label}: {bad} elements differ after {steps} steps", file=sys.stderr)
+        sys.exit(1)
+    del buf
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    value = 2 * nbytes / (elapsed / steps) / 1e9
+    # per kernel class over both directions: each pass reads and writes the tensor once
+    agg = {}
+    passes = {}
+    for which in (0, 1):
+        times = pass_ms[which]
+        if not times:
+            continue
+        ops, _ = E.plan_describe(cases[which][0], cases[which][1], elem, n_sub, alpha)
+        passes["forward" if which == 0 else "inverse"] = [
+            {"kernel": kernel_name(op), "ms": round(sum(t[k] for t in times) / len(times), 4)}
+            for k, op in enumerate(ops)]
+        for k, op in enumerate(ops):
+            ms = sum(t[k] for t in times) / len(times)
+            a = agg.setdefault(kernel_name(op), {"ms": 0.0, "launches": 0})
+            a["ms"] += ms
+            a["launches"] += 1
+    for v in agg.values():
+        v["ms_per_launch"] = v["ms"] / v["launches"]
+        v["GBps"] = 2 * nbytes / (v["ms_per_launch"] / 1e3) / 1e9
+    return {"dims": dims, "perm": perm, "elem_bytes": elem, "alpha": alpha, "n_sub": n_sub,
+            "tensor_bytes": nbytes, "value": value, "ms_per_step": elapsed * 1e3 / steps,
+            "peak_extra_bytes": peak, "peak_extra_frac": peak / nbytes, "per_kernel": agg,
+            "passes": passes}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="c2",
-                    choices=["c2", "c1", "c3", "batch", "serve", "copy"],
-                    help="c2 (default) prints the benchmark line; the others print their own "
-                         "JSON report")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the 6-D secondary block")
+    ap.add_argument("--workload", default="c3",
+                    choices=["c3", "c2", "c1", "batch", "serve", "copy"],
+                    help="c3 (default) prints the benchmark line; c2 prints the same line for "
+                         "the 6-D workload; the others print their own JSON report")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the path runs on one device; "
-              "multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)
+        print("bench.py: this build measures one GPU only (every workload here runs on one "
+              "device; sharding across GPUs is design-only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    if args.steps % 2:
-        args.steps += 1  # forward + inverse pairs
     if args.workload == "c1":
         print(json.dumps(run_single("c1: (2,4096,1024,2) perm 1 3 2 4, uint32 v(i)=i",
                                     [2, 4096, 1024, 2], [1, 3, 2, 4], 4, 0.05, 0, 0, 20)))
-        return
-    if args.workload == "c3":
-        print(json.dumps(run_single("c3: (2,163840,65536,2) perm 1 3 2 4, uint32 seed 2, 172 GB",
-                                    [2, 163840, 65536, 2], [1, 3, 2, 4], 4, 0.05, 1, 2, 2)))
         return
     if args.workload == "batch":
         print(json.dumps(run_batch()))
@@ -384,108 +571,62 @@ def main():
         print(json.dumps(run_serve()))
         return
 
-    import torch
-
-    import eithot_amd as E
-
-    dev = torch.device("cuda:0")
-    vol = 1
-    for d in DIMS:
-        vol *= d
-    nbytes = vol * ELEM
-    buf = torch.empty(nbytes, dtype=torch.uint8, device=dev)
-    E.fill_pattern(buf, vol, ELEM, SEED, 1)
-    torch.cuda.synchronize()
-    inv_dims, inv_perm = inverse(DIMS, PERM)
-    cases = [(DIMS, PERM), (inv_dims, inv_perm)]
-
-    def step(i):
-        d, p = cases[i % 2]
-        E.transpose_tensor(buf, d, p, ELEM, 1, ALPHA)
-
-    for i in range(args.warmup + (args.warmup % 2)):
-        step(i)
-    torch.cuda.synchronize()
-    print(f"bench: warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
-
-    # per-pass HIP events on the launching stream, recorded over the timed steps
-    E.lib().eithot_set_pass_timing(1)
-    import ctypes
-    pass_ms = {0: [], 1: []}
-    peak = 0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step(i)
-        n = E.lib().eithot_last_pass_times(None, 0)
-        arr = (ctypes.c_float * n)()
-        E.lib().eithot_last_pass_times(arr, n)
-        pass_ms[i % 2].append(list(arr))
-        peak = max(peak, E.last_peak_extra_bytes())
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    E.lib().eithot_set_pass_timing(0)
-
-    # parity of the final state: an even number of steps restores the input
-    mism = E.verify_pattern(buf, DIMS, list(range(1, len(DIMS) + 1)), ELEM, SEED, 1)
-    if mism:
-        print(f"bench.py: {mism} elements differ after {args.steps} steps", file=sys.stderr)
-        sys.exit(1)
-
-    ms_per_step = elapsed * 1e3 / args.steps
-    value = 2 * nbytes / (elapsed / args.steps) / 1e9
-
-    # roofline: aggregate per kernel (tile/col/row and map) over both directions
-    agg = {}
-    for which in (0, 1):
-        ops, _ = E.plan_describe(cases[which][0], cases[which][1], ELEM, 1, ALPHA)
-        times = pass_ms[which]
-        for k, op in enumerate(ops):
-            ms = sum(t[k] for t in times) / len(times)
-            name = kernel_name(op)
-            a = agg.setdefault(name, {"ms": 0.0, "launches": 0, "bytes": 0})
-            a["ms"] += ms
-            a["launches"] += 1
-            a["bytes"] += 2 * nbytes  # each pass reads and writes the tensor once
-    dom = max(agg, key=lambda k: agg[k]["ms"])
-    d = agg[dom]
-    achieved = d["bytes"] / (d["ms"] / 1e3) / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_6d_4.5GB.json")
-    if os.path.exists(tf):
-        traffic = json.load(open(tf)).get(dom)
-
+    head = C3 if args.workload == "c3" else C2
+    r = measure(head, args.steps, args.warmup, args.workload)
+    dom = max(r["per_kernel"], key=lambda k: r["per_kernel"][k]["ms"])
+    d = r["per_kernel"][dom]
+    traffic, traffic_src = None, None
+    if head is C3 and os.path.exists(TRAFFIC_FILE):
+        tf = json.load(open(TRAFFIC_FILE))
+        traffic = tf.get("per_launch_bytes", {}).get(dom)
+        traffic_src = os.path.relpath(TRAFFIC_FILE, ROOT) + " (" + tf.get("source", "") + ")"
     out = {
-        "metric": "in-place transpose effective GB/s & % HBM peak; served tensors/s + p99 latency",
-        "value": round(value, 2),
+        "metric": METRIC,
+        "value": round(r["value"], 2),
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(r["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
-        "data": "synthetic (counter-based random uint32, seed 1), resident in HBM",
-        "config": {"workload": "6-D in-place transpose dims (32,48,64,40,36,8) perm (6 4 2 5 1 3), "
-                               "4-byte elements, 4.53 GB; steps alternate perm and inverse",
-                   "dims": DIMS, "perm": PERM, "elem_bytes": ELEM, "alpha": ALPHA,
-                   "hbm_frac_effective": round(value / HBM_PEAK_GBS, 4),
-                   "peak_extra_bytes": peak,
-                   "passes_per_step": {"forward": len(pass_ms[0][0]), "inverse": len(pass_ms[1][0])},
+        "data": f"synthetic (counter-based random uint32, seed {head['seed']}), resident in HBM",
+        "config": {"workload": head["name"], "dims": head["dims"], "perm": head["perm"],
+                   "elem_bytes": head["elem"], "alpha": head["alpha"], "n_sub": r["n_sub"],
+                   "tensor_bytes": r["tensor_bytes"],
+                   "hbm_frac_effective": round(r["value"] / HBM_PEAK_GBS, 4),
+                   "peak_extra_bytes": r["peak_extra_bytes"],
+                   "peak_extra_frac": round(r["peak_extra_frac"], 5),
+                   "passes_per_step": {k: len(v) for k, v in r["passes"].items()},
+                   "checks": "forward result: device closed-form check of every element and "
+                             "4000 sampled elements against the oracle; final state after the "
+                             "timed steps: device closed-form check",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "hip_kernel": HIP_KERNEL.get(dom, dom),
-                     "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic,
-                     "per_kernel": {k: {"ms_per_step": round(v["ms"] / 2, 4),
-                                        "GBps": round(v["bytes"] / (v["ms"] / 1e3) / 1e9, 1)}
-                                    for k, v in agg.items()}},
+                     "achieved": round(d["GBps"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(d["GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "algorithmic_bytes_per_launch": 2 * r["tensor_bytes"],
+                     "ms_per_launch": round(d["ms_per_launch"], 4),
+                     "per_kernel": {k: {"ms_per_launch": round(v["ms_per_launch"], 4),
+                                        "launches_per_2_steps": v["launches"],
+                                        "GBps": round(v["GBps"], 1)}
+                                    for k, v in r["per_kernel"].items()},
+                     "passes": r["passes"]},
     }
-    print(f"bench: timed {args.steps} steps, {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
+    if head is C3 and not args.no_secondary:
+        r2 = measure(C2, args.steps, args.warmup, "c2")
+        out["secondary"] = {
+            "workload": C2["name"], "value": round(r2["value"], 2), "unit": "GB/s",
+            "ms_per_step": round(r2["ms_per_step"], 4),
+            "hbm_frac_effective": round(r2["value"] / HBM_PEAK_GBS, 4),
+            "peak_extra_frac": round(r2["peak_extra_frac"], 5),
+            "per_kernel": {k: {"ms_per_launch": round(v["ms_per_launch"], 4),
+                               "GBps": round(v["GBps"], 1)} for k, v in r2["per_kernel"].items()}}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_threads)
+        out["cpu_baseline"] = cpu_baseline()
         print("bench: cpu baseline done", file=sys.stderr, flush=True)
     print(json.dumps(out))
 

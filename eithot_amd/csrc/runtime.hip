@@ -148,8 +148,10 @@ void Device::trim_locked(int64_t budget, const Plan* keep) {
   int64_t others = 0;
   for (const PlanPtr& p : order)
     if (p.get() != keep) others += p->device_table_bytes(id);
+  // the arena this call ends with: the current one, or one grown to its need
   const int64_t need = keep ? keep->scratch_bytes : 0;
-  if (arena_bytes + keep_tables + others <= budget) return;
+  auto held_after = [&] { return std::max(arena_bytes, need) + keep_tables + others; };
+  if (held_after() <= budget) return;
   // an arena larger than this call needs is given back first
   if (arena && arena_bytes > need) {
     if (arena_ready) (void)hipEventSynchronize(arena_ready);
@@ -159,7 +161,7 @@ void Device::trim_locked(int64_t budget, const Plan* keep) {
   }
   // then the tables of other plans, least recently used first
   for (auto it = order.rbegin(); it != order.rend(); ++it) {
-    if (arena_bytes + keep_tables + others <= budget) break;
+    if (held_after() <= budget) break;
     if (it->get() == keep) continue;
     const int64_t b = (*it)->device_table_bytes(id);
     if (!b) continue;

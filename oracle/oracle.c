@@ -105,6 +105,10 @@ int or_permute(const void* in, void* out, const int64_t* dims, const int* perm0,
  * (contiguous in the output) form a plane that is transposed in 32x32
  * tiles; all other dims are an outer odometer split across threads. */
 typedef struct {
+  uint64_t lo, hi;
+} or_u128;
+
+typedef struct {
   const unsigned char* in;
   unsigned char* out;
   const int64_t* dims;
@@ -140,10 +144,25 @@ static void* mt_worker(void* arg) {
     int64_t o0 = ob * T, o1 = o0 + T < dO ? o0 + T : dO;
     for (int64_t i0 = 0; i0 < d0; i0 += T) {
       int64_t i1 = i0 + T < d0 ? i0 + T : d0;
-      for (int64_t y = o0; y < o1; ++y)
-        for (int64_t x = i0; x < i1; ++x)
-          memcpy(J->out + (obase + x * os0 + y) * elem, J->in + (ibase + x + y * isO) * elem,
-                 (size_t)elem);
+      /* element copies of a width known at compile time for the common sizes */
+#define OR_TILE(TYPE)                                                              \
+  for (int64_t y = o0; y < o1; ++y)                                                \
+    for (int64_t x = i0; x < i1; ++x) {                                            \
+      TYPE v;                                                                      \
+      memcpy(&v, J->in + (ibase + x + y * isO) * elem, sizeof(TYPE));              \
+      memcpy(J->out + (obase + x * os0 + y) * elem, &v, sizeof(TYPE));             \
+    }
+      if (elem == 1) { OR_TILE(uint8_t) }
+      else if (elem == 2) { OR_TILE(uint16_t) }
+      else if (elem == 4) { OR_TILE(uint32_t) }
+      else if (elem == 8) { OR_TILE(uint64_t) }
+      else if (elem == 16) { OR_TILE(or_u128) }
+      else
+        for (int64_t y = o0; y < o1; ++y)
+          for (int64_t x = i0; x < i1; ++x)
+            memcpy(J->out + (obase + x * os0 + y) * elem, J->in + (ibase + x + y * isO) * elem,
+                   (size_t)elem);
+#undef OR_TILE
     }
   }
   return NULL;
@@ -152,6 +171,23 @@ static void* mt_worker(void* arg) {
 int or_permute_mt(const void* in, void* out, const int64_t* dims, const int* perm0, int rank,
                   int elem, int nthreads) {
   if (check_perm(perm0, rank)) return -1;
+  /* leading dims that stay in place move with the element: fold them into
+   * it (up to 4 KiB), so that the tiled plane is formed by dims that move */
+  int m = 0;
+  int64_t fold = 1;
+  while (m < rank - 1 && perm0[m] == m && (int64_t)elem * fold * dims[m] <= 4096) {
+    fold *= dims[m];
+    ++m;
+  }
+  if (m > 0) {
+    int64_t d2[OR_MAX_RANK];
+    int p2[OR_MAX_RANK];
+    for (int k = m; k < rank; ++k) {
+      d2[k - m] = dims[k];
+      p2[k - m] = perm0[k] - m;
+    }
+    return or_permute_mt(in, out, d2, p2, rank - m, (int)(elem * fold), nthreads);
+  }
   if (nthreads < 1) nthreads = 1;
   if (nthreads > 256) nthreads = 256;
   mt_job base;

@@ -45,6 +45,18 @@ def test_permute_mt_matches_single_thread():
         a = O.permute(arr, dims, perm, elem)
         b = O.permute(arr, dims, perm, elem, nthreads=4)
         assert np.array_equal(a, b), (dims, perm, elem)
+    # leading dims that stay in place (folded into the element), also past
+    # the fold limit
+    for dims, perm in (([2, 300, 500, 2], [0, 2, 1, 3]), ([3, 5, 70, 90], [0, 1, 3, 2]),
+                       ([4000, 3, 7], [0, 2, 1]), ([2, 2, 2, 2, 9, 11], [0, 1, 2, 5, 4, 3]),
+                       ([6, 9], [0, 1])):
+        for elem in (1, 8):
+            arr = O.fill(int(np.prod(dims)), elem, 4, 1)
+            a = O.permute(arr, dims, perm, elem)
+            assert np.array_equal(a, np_reference(arr.view({1: np.uint8, 8: np.uint64}[elem]),
+                                                  dims, perm).view(np.uint8)), (dims, perm)
+            b = O.permute(arr, dims, perm, elem, nthreads=3)
+            assert np.array_equal(a, b), (dims, perm, elem)
 
 
 def test_known_answer_closed_form():
