@@ -40,9 +40,10 @@ C2 = {"name": "6-D high-order in-place transpose dims (32,48,64,40,36,8) perm (6
               "inverse",
       "dims": [32, 48, 64, 40, 36, 8], "perm": [6, 4, 2, 5, 1, 3], "elem": 4, "alpha": 0.05,
       "seed": 1}
-# HBM read+write traffic per launch of each kernel class on the headline
-# workload, from the FETCH_SIZE / WRITE_SIZE summary committed beside it
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_4d_172GB.json")
+# HBM read+write traffic per launch of each kernel class, from the
+# FETCH_SIZE / WRITE_SIZE summaries committed beside these files
+C3["traffic"] = os.path.join(ROOT, "profiles", "traffic_4d_172GB.json")
+C2["traffic"] = os.path.join(ROOT, "profiles", "traffic_6d_4.5GB.json")
 
 
 def inverse(dims, perm1):
@@ -576,10 +577,10 @@ def main():
     dom = max(r["per_kernel"], key=lambda k: r["per_kernel"][k]["ms"])
     d = r["per_kernel"][dom]
     traffic, traffic_src = None, None
-    if head is C3 and os.path.exists(TRAFFIC_FILE):
-        tf = json.load(open(TRAFFIC_FILE))
+    if os.path.exists(head["traffic"]):
+        tf = json.load(open(head["traffic"]))
         traffic = tf.get("per_launch_bytes", {}).get(dom)
-        traffic_src = os.path.relpath(TRAFFIC_FILE, ROOT) + " (" + tf.get("source", "") + ")"
+        traffic_src = os.path.relpath(head["traffic"], ROOT) + " (" + tf.get("source", "") + ")"
     out = {
         "metric": METRIC,
         "value": round(r["value"], 2),

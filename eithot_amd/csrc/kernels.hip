@@ -1346,7 +1346,8 @@ inline bool cycle_small_ok(int variant) { return variant == 0 || variant >= 4; }
 // cells in `scratch`; *w receives the word size, *variant the kernel shape
 // it can use.  *empty is set when the pass moves nothing (fixed points only).
 hipError_t prepare_cycle_args(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
-                              int* variant, CycArgs& a, int64_t* w_out, bool* empty) {
+                              int* variant, CycArgs& a, int64_t* w_out, bool* empty,
+                              const void* table = nullptr) {
   *empty = false;
   if (!op.cyc || (uint64_t)op.R * (uint64_t)op.C >= (1ull << 32)) return hipErrorInvalidValue;
   const CycleTable& t = *op.cyc;
@@ -1356,8 +1357,8 @@ hipError_t prepare_cycle_args(const Op& op, char* data, char* scratch, int64_t s
   }
   const uint64_t need = (uint64_t)op.B * t.slots.size() * (uint64_t)op.U;
   if (need && (!scratch || (uint64_t)scratch_bytes < need)) return hipErrorInvalidValue;
-  hipError_t e;
-  const char* d = (const char*)cycle_table_device(t, &e);
+  hipError_t e = hipSuccess;
+  const char* d = table ? (const char*)table : (const char*)cycle_table_device(t, &e);
   if (!d) return e;
   const int64_t w = word_bytes(op, (uintptr_t)data);
   *w_out = w;
@@ -1397,11 +1398,12 @@ hipError_t prepare_cycle_args(const Op& op, char* data, char* scratch, int64_t s
 }
 
 hipError_t launch_cycle(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
-                        int variant, hipStream_t stream) {
+                        int variant, hipStream_t stream, const void* table) {
   CycArgs a;
   int64_t w = 1;
   bool empty = false;
-  hipError_t e = prepare_cycle_args(op, data, scratch, scratch_bytes, &variant, a, &w, &empty);
+  hipError_t e =
+      prepare_cycle_args(op, data, scratch, scratch_bytes, &variant, a, &w, &empty, table);
   if (e != hipSuccess || empty) return e;
   switch (w) {
     case 16: return launch_cycle_w<W16>(a, variant, stream);
@@ -1586,14 +1588,15 @@ hipError_t launch_block_w(BlkArgs a, int vec, int64_t lds, int64_t groups, const
 // Arguments of a fused block pass; *w: word size, *vec: packed stores,
 // *lds: LDS bytes per workgroup, *groups: workgroups of K blocks.
 hipError_t prepare_block_args(const Op& op, char* data, const LaunchCfg& cfg, BlkArgs& a,
-                              int64_t* w_out, int* vec_out, int64_t* lds_out, int64_t* groups_out) {
+                              int64_t* w_out, int* vec_out, int64_t* lds_out, int64_t* groups_out,
+                              const void* table = nullptr) {
   if (!op.blk || op.blk->cells != op.R || op.blk->cell_bytes != op.U) return hipErrorInvalidValue;
   const BlockTable& t = *op.blk;
   const int64_t F = op.R * op.U;
   if (F > 160 * 1024 || op.R >= (1ll << 31) || op.B <= 0) return hipErrorInvalidValue;
   hipError_t e = hipSuccess;
   const void* d = nullptr;
-  if (!t.ndig && !(d = block_table_device(t, &e))) return e;
+  if (!t.ndig && !(d = table ? table : block_table_device(t, &e))) return e;
   const uintptr_t base = (uintptr_t)(data + op.base);
   int64_t w = 16;
   while (w > 1 && ((op.U % w) || (base % w) || (F % w))) w /= 2;
@@ -1647,11 +1650,12 @@ hipError_t prepare_block_args(const Op& op, char* data, const LaunchCfg& cfg, Bl
   return hipSuccess;
 }
 
-hipError_t launch_block(const Op& op, char* data, const LaunchCfg& cfg, hipStream_t stream) {
+hipError_t launch_block(const Op& op, char* data, const LaunchCfg& cfg, hipStream_t stream,
+                        const void* table) {
   BlkArgs a;
   int64_t w = 1, lds = 0, groups = 0;
   int vec = 0;
-  hipError_t e = prepare_block_args(op, data, cfg, a, &w, &vec, &lds, &groups);
+  hipError_t e = prepare_block_args(op, data, cfg, a, &w, &vec, &lds, &groups, table);
   if (e != hipSuccess) return e;
   switch (w) {
     case 16: return launch_block_w<W16>(a, vec, lds, groups, cfg, stream);
@@ -2116,10 +2120,10 @@ hipError_t prepare_op(const Op& op) {
 }
 
 hipError_t launch_op(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
-                     const LaunchCfg& cfg, hipStream_t stream) {
+                     const LaunchCfg& cfg, hipStream_t stream, const void* table) {
   if (op.kind == OP_CYCLE)
-    return launch_cycle(op, data, scratch, scratch_bytes, cfg.cycle_variant, stream);
-  if (op.kind == OP_BLOCK) return launch_block(op, data, cfg, stream);
+    return launch_cycle(op, data, scratch, scratch_bytes, cfg.cycle_variant, stream, table);
+  if (op.kind == OP_BLOCK) return launch_block(op, data, cfg, stream, table);
   if (op.kind == OP_SEGTILE) return launch_segtile(op, data, cfg, stream);
   if (op.kind == 0 || op.us <= 0 || op.U % op.us) return hipErrorInvalidValue;
   if (needs64(op)) return run_op<uint64_t>(op, data, scratch, scratch_bytes, cfg, stream);

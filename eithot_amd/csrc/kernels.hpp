@@ -24,9 +24,13 @@ struct LaunchCfg {
 };
 
 // One pass of `op` over the tensor at `data`, on `stream`.  Staged ops use
-// `scratch` (scratch_bytes >= one footprint).  Returns a hipError_t.
+// `scratch` (scratch_bytes >= one footprint).  `table`, when given, is the
+// device copy of the op's cycle or block table to read (laid out as the
+// launcher's own upload: segments then slots; uint16 indices when a block
+// has at most 65536 cells, else uint32); otherwise the launcher uploads the
+// table on first use.  Returns a hipError_t.
 hipError_t launch_op(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
-                     const LaunchCfg& cfg, hipStream_t stream);
+                     const LaunchCfg& cfg, hipStream_t stream, const void* table = nullptr);
 
 // Uploads what an op needs on the current device before its first launch
 // (the cycle table of OP_CYCLE; nothing for other kinds).  Call before

@@ -98,16 +98,21 @@ LaunchCfg Device::launch_cfg() const {
   return c;
 }
 
-PlanPtr Device::plan_for(const int64_t* dims, const int* perm1, int rank, size_t elem, int n_sub,
-                         double alpha, int* rc) {
-  const DeviceProfile pr = profile();
+std::string plan_key(const int64_t* dims, const int* perm1, int rank, size_t elem, int n_sub,
+                     double alpha) {
   std::ostringstream k;
   k << tuning_get().epoch << "#";
   for (int i = 0; i < rank; ++i) k << dims[i] << ",";
   k << "|";
   for (int i = 0; i < rank; ++i) k << perm1[i] << ",";
   k << "|" << elem << "|" << n_sub << "|" << alpha;
-  const std::string key = k.str();
+  return k.str();
+}
+
+PlanPtr Device::plan_for(const int64_t* dims, const int* perm1, int rank, size_t elem, int n_sub,
+                         double alpha, int* rc) {
+  const DeviceProfile pr = profile();
+  const std::string key = plan_key(dims, perm1, rank, elem, n_sub, alpha);
   {
     std::lock_guard<std::mutex> lk(mu);
     if (PlanPtr p = plans.get(key)) {

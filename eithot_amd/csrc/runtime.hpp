@@ -59,6 +59,9 @@ struct Device {
   int64_t held_bytes();
 };
 
+// Cache key of a plan: the arguments and the tuning epoch.
+std::string plan_key(const int64_t* dims, const int* perm1, int rank, size_t elem, int n_sub,
+                     double alpha);
 Device* device_ctx(int* err);
 int fail(int code, const char* fmt, ...);
 void set_peak(size_t bytes);
