@@ -413,6 +413,63 @@ def run_serve(count=96, rates=(100, 300, 1000, 3000)):
             "max_inflight": 16, "streams": 4, "sweep": results}
 
 
+def pass_class(op, data_ptr):
+    """Kernel class of a pass: kind, map, staged, and the word width its
+    kernel moves (the widest power of two up to 16 dividing the slice, the
+    unit and the tensor's address)."""
+    w = 16
+    while w > 1 and (op["us"] % w or op["U"] % w or (data_ptr + op["base"]) % w):
+        w //= 2
+    return f"{kernel_name(op)}/w{w}"
+
+
+def run_census(which="serve", count=None):
+    """Every pass of every request of the served (or batched) workload,
+    timed alone with HIP events on the launching stream: GB/s per pass class
+    (2 x tensor bytes per pass over its duration).  Each tensor is filled,
+    transposed once and checked on the device."""
+    import numpy as np
+    import torch
+
+    import eithot_amd as E
+    from eithot_amd.workloads import batch_specs, serve_specs
+    specs = serve_specs(count=count or 96) if which == "serve" else batch_specs(count=count or 4096)
+    cap = max(_vol(d) * e for d, _, e in specs)
+    buf = torch.empty(cap, dtype=torch.uint8, device="cuda:0")
+    E.lib().eithot_set_pass_timing(1)
+    agg = {}
+    for k, (dims, perm, elem) in enumerate(specs):
+        vol = _vol(dims)
+        E.fill_pattern(buf, vol, elem, 5000 + k, 1)
+        E.transpose_tensor(buf, dims, perm, elem, 1, 0.05)  # plans; tables uploaded
+        E.fill_pattern(buf, vol, elem, 5000 + k, 1)
+        E.transpose_tensor(buf, dims, perm, elem, 1, 0.05)
+        n = E.lib().eithot_last_pass_times(None, 0)
+        arr = (ctypes.c_float * n)()
+        E.lib().eithot_last_pass_times(arr, n)
+        if E.verify_pattern(buf, dims, perm, elem, 5000 + k, 1):
+            print(f"census: request {k} {dims} {perm} differs", file=sys.stderr)
+            sys.exit(1)
+        ops, _ = E.plan_describe(dims, perm, elem, 1, 0.05)
+        for op, ms in zip(ops, arr):
+            c = agg.setdefault(pass_class(op, buf.data_ptr()), {"passes": 0, "ms": 0.0, "bytes": 0})
+            c["passes"] += 1
+            c["ms"] += ms
+            c["bytes"] += 2 * vol * elem
+        if k % 16 == 15:
+            print(f"census {which}: {k + 1} of {len(specs)}", file=sys.stderr, flush=True)
+    E.lib().eithot_set_pass_timing(0)
+    out = {k: {"passes": v["passes"], "ms": round(v["ms"], 3), "GB": round(v["bytes"] / 1e9, 2),
+               "GBps": round(v["bytes"] / (v["ms"] / 1e3) / 1e9, 1),
+               "frac": round(v["bytes"] / (v["ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 3)}
+           for k, v in sorted(agg.items(), key=lambda kv: -kv[1]["ms"])}
+    tot_ms = sum(v["ms"] for v in agg.values())
+    tot_b = sum(v["bytes"] for v in agg.values())
+    return {"workload": f"pass census of the {which} workload (each request transposed alone)",
+            "requests": len(specs), "total_ms": round(tot_ms, 2),
+            "GBps_all_passes": round(tot_b / (tot_ms / 1e3) / 1e9, 1), "classes": out}
+
+
 def run_copy(nbytes=1 << 30, reps=5):
     """Plain device-to-device copy of a known size: the calibration kernel for
     the HBM byte counters (each launch reads and writes nbytes)."""
@@ -550,7 +607,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-secondary", action="store_true", help="skip the 6-D secondary block")
     ap.add_argument("--workload", default="c3",
-                    choices=["c3", "c2", "c1", "batch", "serve", "copy"],
+                    choices=["c3", "c2", "c1", "batch", "serve", "copy", "census-serve",
+                             "census-batch"],
                     help="c3 (default) prints the benchmark line; c2 prints the same line for "
                          "the 6-D workload; the others print their own JSON report")
     args = ap.parse_args()
@@ -570,6 +628,9 @@ def main():
         return
     if args.workload == "serve":
         print(json.dumps(run_serve()))
+        return
+    if args.workload.startswith("census-"):
+        print(json.dumps(run_census(args.workload.split("-")[1])))
         return
 
     head = C3 if args.workload == "c3" else C2
