@@ -24,7 +24,8 @@ ERRORS = {
 
 # every symbol include/eithot.h declares
 EXPORTS = [
-    "eithot_transpose", "eithot_transpose_async", "eithot_inplace_transpose",
+    "eithot_transpose", "eithot_transpose_async", "eithot_transpose_noalloc",
+    "eithot_inplace_transpose",
     "eithot_last_peak_extra_bytes", "eithot_last_error", "eithot_release", "eithot_estimate",
     "eithot_estimate_reference", "eithot_decompose", "eithot_plan_describe",
     "eithot_plan_partition", "eithot_plan_block_table", "eithot_plan_cycle_maps",
@@ -78,6 +79,8 @@ def lib():
     L.eithot_transpose.argtypes = [vp, i64p, ip, ctypes.c_int, sz, ctypes.c_int, ctypes.c_double]
     L.eithot_transpose_async.argtypes = [vp, i64p, ip, ctypes.c_int, sz, ctypes.c_int,
                                          ctypes.c_double, vp]
+    L.eithot_transpose_noalloc.argtypes = [vp, i64p, ip, ctypes.c_int, sz, ctypes.c_int,
+                                           ctypes.c_double]
     L.eithot_inplace_transpose.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip, ip, sz,
                                            ctypes.c_int, ctypes.c_double]
     L.eithot_last_peak_extra_bytes.restype = sz

@@ -61,6 +61,18 @@ int eithot_transpose(void* dev, const int64_t* dims, const int* perm, int order,
 int eithot_transpose_async(void* dev, const int64_t* dims, const int* perm, int order,
                            size_t elem_bytes, int n_sub, double alpha, void* stream);
 
+/* Same as eithot_transpose, but allocates no device memory: no counterpart
+ * in the reference; for callers that cannot allocate (tight serving loops,
+ * harnesses that forbid hipMalloc).  The plan's scratch and the device
+ * copies of its pass tables live in one static device array of 64 MiB per
+ * GPU, held until the process ends; a plan that needs more is refused with
+ * EITHOT_ERR_NOMEM before anything is launched.  Synchronous on the null
+ * stream.  Calls on one GPU share the array: calls that overlap on one GPU
+ * race (serialise them).  eithot_last_peak_extra_bytes() afterwards gives
+ * the bytes of the array the call used. */
+int eithot_transpose_noalloc(void* dev, const int64_t* dims, const int* perm, int order,
+                             size_t elem_bytes, int n_sub, double alpha);
+
 /* The reference library's own signature and semantics: 0-based permutation,
  * int dims, element offset `source`, sizeofType 1/2/4/8, num_block and the
  * extra-memory ratio.  Replaces inplace::transpose
