@@ -70,6 +70,14 @@ def lib():
         raise RuntimeError(
             f"{LIB_PATH} is missing: build the HIP extension first "
             "(python -c 'import __graft_entry__ as g; g.build()')")
+    # One HIP runtime per process: PyTorch ships its own, and a process that
+    # loads libeithot.so first and torch afterwards ends up with two, one of
+    # which finds no device.  Torch, when installed, is loaded first so the
+    # library binds to the runtime torch's tensors come from.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     i64p = ctypes.POINTER(ctypes.c_int64)
     ip = ctypes.POINTER(ctypes.c_int)

@@ -138,7 +138,8 @@ int eithot_batch_build(eithot_batch* b, double alpha, int tune) {
   if (b->built) return fail(EITHOT_ERR_INVALID, "batch already built");
   if (!(alpha > 0)) return fail(EITHOT_ERR_INVALID, "alpha must be > 0");
   const size_t n = b->tensors.size();
-  const DeviceProfile prof = b->dev->profile();
+  DeviceProfile prof = b->dev->profile();
+  prof.cycle_shift = 0;  // batched cycle items move narrow cells word by word
   b->cfg = b->dev->launch_cfg();
   b->fwd.resize(n);
   b->inv.resize(n);

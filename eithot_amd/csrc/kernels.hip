@@ -107,7 +107,14 @@ struct Args {
   int cps_log2;      // log2(cps) when cps is a power of two, else -1
   uint64_t fp0, G;   // staged: first footprint and count of the group
   int contiguous;    // S == 1 and the K footprints are one contiguous byte range
-  typename FDsel<Idx>::type fS, fR, fC, fW, fP, fwu, fcpr, fld, fst, fvpr, fcps;
+  // wide path (column and row passes of whole units narrower than 16-byte
+  // words): global rows move as aligned 16-byte chunks into an LDS copy
+  // that keeps each row's address modulo 16; stores are 16-byte chunks
+  // gathered word by word.  hi: end of the pass's bytes (offset from data).
+  int wide;
+  Idx ncw;           // column pass: 16-byte chunks per LDS row (rs / 16)
+  uint64_t hi;
+  typename FDsel<Idx>::type fS, fR, fC, fW, fP, fwu, fcpr, fld, fst, fvpr, fcps, fncw, fU, fCU;
   GridMaps<Idx> g;
 };
 
@@ -308,11 +315,120 @@ __device__ __forceinline__ void tile_body(const Args<Idx>& a, const Idx f0, unsi
 // column pass: panels of W columns x R rows; LDS row i of footprint k holds
 // the panel's segment (W*us bytes) at k*fpl + i*rs
 // ---------------------------------------------------------------------------
+// 16-byte global load / store of an aligned chunk
+__device__ __forceinline__ W16 ld16(const char* p) { return *reinterpret_cast<const W16*>(p); }
+__device__ __forceinline__ void st16(char* p, const W16& v) { *reinterpret_cast<W16*>(p) = v; }
+
+// Wide column pass.  Row i of panel k (global bytes [g, g + L), g = f.addr +
+// i*CU, L = weff*us: W whole units, or one us-byte slice of a unit when
+// W == 1) lies in LDS at k*fpl + i*rs + (address of g mod 16): the
+// aligned 16-byte chunks that cover the row land on aligned LDS chunks.  A
+// chunk wholly inside the pass's bytes moves with one 16-byte load (bytes
+// outside the row are loaded but never stored); a chunk at the pass's edge
+// moves word by word.  Stores: output chunk c of row i is gathered word by
+// word (column jj's words come from LDS row col_src(i, j0 + jj)) and stored
+// with one 16-byte store when it lies inside the row, word by word at the
+// row's two ends.
+template <typename Word, typename Idx, int MAP>
+__device__ __forceinline__ void col_wide(const Args<Idx>& a, const Idx f0, const Idx nf,
+                                         unsigned char* lds) {
+  constexpr Idx w = sizeof(Word);
+  constexpr int V = 16 / (int)sizeof(Word);
+  const Idx tid = threadIdx.x, nt = blockDim.x;
+  const uintptr_t d0 = (uintptr_t)a.data;
+  const Idx n = nf * a.ld_per;  // ld_per = R * ncw
+  for (Idx t0 = tid; t0 < n; t0 += nt * UNR) {
+    W16 v[UNR];
+    Idx off[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const Idx t = t0 + (Idx)u * nt;
+      off[u] = ~(Idx)0;
+      if (t >= n) continue;
+      const Idx k = a.fld.div(t);
+      const Idx rem = t - k * a.ld_per;
+      const Idx i = a.fncw.div(rem);
+      const Idx c = rem - i * a.ncw;
+      const Foot<Idx, OP_COL> f = decode<Idx, OP_COL>(a, f0 + k);
+      const int64_t g = (int64_t)(f.addr + (uint64_t)i * a.CU);
+      const int64_t L = (int64_t)f.weff * (int64_t)a.us;
+      // chunk start, relative to data: may lie up to 15 bytes before it
+      const int64_t cs = g - (int64_t)((d0 + (uint64_t)g) & 15) + (int64_t)c * 16;
+      if (cs >= g + L) continue;
+      const Idx lo = k * a.fpl + i * a.rs + c * 16;
+      if (cs >= (int64_t)a.base && cs + 16 <= (int64_t)a.hi) {
+        v[u] = ld16(a.data + cs);
+        off[u] = lo;
+      } else {
+        const int64_t b0 = cs < g ? g : cs, b1 = cs + 16 < g + L ? cs + 16 : g + L;
+        for (int64_t b = b0; b < b1; b += w)
+          *reinterpret_cast<Word*>(lds + lo + (b - cs)) = *reinterpret_cast<const Word*>(a.data + b);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u)
+      if (off[u] != ~(Idx)0) *reinterpret_cast<W16*>(lds + off[u]) = v[u];
+  }
+  __syncthreads();
+  for (Idx t = tid; t < n; t += nt) {
+    const Idx k = a.fld.div(t);
+    const Idx rem = t - k * a.ld_per;
+    const Idx i = a.fncw.div(rem);
+    const Idx c = rem - i * a.ncw;
+    const Foot<Idx, OP_COL> f = decode<Idx, OP_COL>(a, f0 + k);
+    const int64_t g = (int64_t)(f.addr + (uint64_t)i * a.CU);
+    const int64_t L = (int64_t)f.weff * (int64_t)a.us;
+    const int64_t cs = g - (int64_t)((d0 + (uint64_t)g) & 15) + (int64_t)c * 16;
+    if (cs >= g + L) continue;
+    const int64_t b0 = cs < g ? g : cs, b1 = cs + 16 < g + L ? cs + 16 : g + L;
+    // word b0 of the row: column jj, byte xu of its unit (or slice)
+    const Idx x = (Idx)(b0 - g);
+    Idx jj = a.fU.div(x);
+    Idx xu = x - jj * (Idx)a.us;
+    Idx s = col_src<Idx, MAP>(a, i, f.row + jj);
+    // LDS byte of (row r, byte y of the row)
+    auto at = [&](Idx r, Idx y) -> const Word* {
+      const Idx sh = (Idx)((d0 + f.addr + (uint64_t)r * a.CU) & 15);
+      return reinterpret_cast<const Word*>(lds + k * a.fpl + r * a.rs + sh + y);
+    };
+    if (b0 == cs && b1 == cs + 16) {
+      Packed<Word, V> tmp;
+#pragma unroll
+      for (int q = 0; q < V; ++q) {
+        if (xu == (Idx)a.us) {
+          xu = 0;
+          ++jj;
+          s = col_src<Idx, MAP>(a, i, f.row + jj);
+        }
+        tmp.e[q] = *at(s, jj * (Idx)a.us + xu);
+        xu += w;
+      }
+      *reinterpret_cast<typename Packed<Word, V>::P*>(a.data + cs) = tmp.p;
+    } else {
+      for (int64_t b = b0; b < b1; b += w) {
+        if (xu == (Idx)a.us) {
+          xu = 0;
+          ++jj;
+          s = col_src<Idx, MAP>(a, i, f.row + jj);
+        }
+        *reinterpret_cast<Word*>(a.data + b) = *at(s, jj * (Idx)a.us + xu);
+        xu += w;
+      }
+    }
+  }
+}
+
 template <typename Word, typename Idx, int MAP, int VEC>
 __device__ __forceinline__ void col_body(const Args<Idx>& a, const Idx f0, unsigned char* lds) {
   const Idx nf = (a.nfp - f0) < a.K ? (a.nfp - f0) : a.K;
   const Idx tid = threadIdx.x, nt = blockDim.x;
   constexpr Idx w = sizeof(Word);
+  if constexpr (sizeof(Word) < 16) {
+    if (a.wide) {
+      col_wide<Word, Idx, MAP>(a, f0, nf, lds);
+      return;
+    }
+  }
 
   {
     // a.cpr != 0: every panel is full and its segment is whole 16-byte chunks
@@ -396,11 +512,98 @@ __device__ __forceinline__ void col_body(const Args<Idx>& a, const Idx f0, unsig
 // ---------------------------------------------------------------------------
 // row pass: K rows of C cells per workgroup; LDS row k at k*rs
 // ---------------------------------------------------------------------------
+// Wide row pass (whole units, S == 1): the workgroup's nf rows are one
+// contiguous run [g0, g0 + nf*CU), kept in LDS at its address modulo 16, so
+// that the run's aligned 16-byte chunks land on aligned LDS chunks.  Loads
+// and stores as in the wide column pass; output word b of the run is byte
+// xu of cell j of row k, taken from cell row_src(i_k, j) of the same row.
+template <typename Word, typename Idx, int MAP>
+__device__ __forceinline__ void row_wide(const Args<Idx>& a, const Idx f0, const Idx nf,
+                                         unsigned char* lds) {
+  constexpr Idx w = sizeof(Word);
+  constexpr int V = 16 / (int)sizeof(Word);
+  const Idx tid = threadIdx.x, nt = blockDim.x;
+  const uintptr_t d0 = (uintptr_t)a.data;
+  const int64_t g0 = (int64_t)(a.base + (uint64_t)f0 * a.CU);
+  const int64_t g1 = g0 + (int64_t)nf * (int64_t)a.CU;
+  // first chunk, relative to data: may lie up to 15 bytes before it
+  const int64_t a0 = g0 - (int64_t)((d0 + (uint64_t)g0) & 15);
+  const Idx sh = (Idx)(g0 - a0);
+  const Idx n = (Idx)((g1 - a0 + 15) / 16);
+  for (Idx t0 = tid; t0 < n; t0 += nt * UNR) {
+    W16 v[UNR];
+    bool full[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const Idx t = t0 + (Idx)u * nt;
+      full[u] = false;
+      if (t >= n) continue;
+      const int64_t cs = a0 + (int64_t)t * 16;
+      if (cs >= (int64_t)a.base && cs + 16 <= (int64_t)a.hi) {
+        v[u] = ld16(a.data + cs);
+        full[u] = true;
+      } else {
+        const int64_t b0 = cs < g0 ? g0 : cs, b1 = cs + 16 < g1 ? cs + 16 : g1;
+        for (int64_t b = b0; b < b1; b += w)
+          *reinterpret_cast<Word*>(lds + (b - a0)) = *reinterpret_cast<const Word*>(a.data + b);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u)
+      if (full[u]) *reinterpret_cast<W16*>(lds + (t0 + (Idx)u * nt) * 16) = v[u];
+  }
+  __syncthreads();
+  for (Idx t = tid; t < n; t += nt) {
+    const int64_t cs = a0 + (int64_t)t * 16;
+    const int64_t b0 = cs < g0 ? g0 : cs, b1 = cs + 16 < g1 ? cs + 16 : g1;
+    const Idx x = (Idx)(b0 - g0);
+    Idx k = a.fCU.div(x);
+    const Idx xr = x - k * (Idx)a.CU;
+    Idx j = a.fU.div(xr);
+    Idx xu = xr - j * (Idx)a.U;
+    Idx i = a.fR.mod(f0 + k);
+    Idx src = row_src<Idx, MAP>(a, i, j);
+    auto next = [&]() {
+      xu += w;
+      if (xu == (Idx)a.U) {
+        xu = 0;
+        if (++j == a.C) {
+          j = 0;
+          ++k;
+          i = a.fR.mod(f0 + k);
+        }
+        if (k < nf) src = row_src<Idx, MAP>(a, i, j);
+      }
+    };
+    if (b0 == cs && b1 == cs + 16) {
+      Packed<Word, V> tmp;
+#pragma unroll
+      for (int q = 0; q < V; ++q) {
+        tmp.e[q] = *reinterpret_cast<const Word*>(lds + sh + k * (Idx)a.CU + src * (Idx)a.U + xu);
+        next();
+      }
+      *reinterpret_cast<typename Packed<Word, V>::P*>(a.data + cs) = tmp.p;
+    } else {
+      for (int64_t b = b0; b < b1; b += w) {
+        *reinterpret_cast<Word*>(a.data + b) =
+            *reinterpret_cast<const Word*>(lds + sh + k * (Idx)a.CU + src * (Idx)a.U + xu);
+        next();
+      }
+    }
+  }
+}
+
 template <typename Word, typename Idx, int MAP, int VEC>
 __device__ __forceinline__ void row_body(const Args<Idx>& a, const Idx f0, unsigned char* lds) {
   const Idx nf = (a.nfp - f0) < a.K ? (a.nfp - f0) : a.K;
   const Idx tid = threadIdx.x, nt = blockDim.x;
   constexpr Idx w = sizeof(Word);
+  if constexpr (sizeof(Word) < 16) {
+    if (a.wide) {
+      row_wide<Word, Idx, MAP>(a, f0, nf, lds);
+      return;
+    }
+  }
 
   if (a.cpr) {
     // S == 1: the nf rows are one contiguous byte range (rs = C*U, no padding)
@@ -732,6 +935,25 @@ struct Prepared {
   int64_t groups = 0;
 };
 
+// Column and row passes over whole units narrower than 16-byte words take
+// the wide path, unless their rows are whole aligned 16-byte chunks of
+// one-word units (then 16-byte loads and packed stores apply as they are).
+// The choice depends on the op only (its offsets as if the tensor were
+// 16-byte aligned), so the LDS size is the same for any tensor address.
+constexpr int64_t kWideLdsMax = 160 * 1024;
+inline bool wide_ok(const Op& op) {
+  if (op.staged) return false;
+  if (op.kind != OP_COL && op.kind != OP_ROW) return false;
+  // rows of a column pass may be one slice of a unit (one column per panel)
+  if (op.us != op.U && !(op.kind == OP_COL && op.W == 1)) return false;
+  const int64_t w = word_bytes(op, 0);
+  if (w >= 16) return false;
+  const int64_t V = 16 / w;
+  const bool aligned = op.base % 16 == 0 && (op.C * op.U) % 16 == 0 && op.U == w && op.us == op.U;
+  if (op.kind == OP_ROW) return !(aligned && op.C % V == 0);
+  return !(aligned && op.W % V == 0 && op.C % op.W == 0);
+}
+
 // Fills every LDS-kernel field of `a` for op (not staged) under cfg.
 template <typename Idx>
 void prepare_lds(const Op& op, char* data, const LaunchCfg& cfg, Args<Idx>& a, Prepared& p) {
@@ -770,6 +992,19 @@ void prepare_lds(const Op& op, char* data, const LaunchCfg& cfg, Args<Idx>& a, P
     a.ld_per = chunk16 ? a.R * a.cpr : a.R * a.W * a.wu;
     a.vpr = vec ? (Idx)(op.W / V) : 1;
     a.st_per = vec ? a.R * a.vpr : a.R * a.W * a.wu;
+    // whole units that do not move as 16-byte chunks take the wide path
+    // when its LDS rows (16 bytes of slack each) fit; the choice depends on
+    // the op only, so the LDS size is the same for any tensor address
+    const int64_t rsw = (seg + 15 + 15) / 16 * 16;
+    if (wide_ok(op) && op.R * rsw <= kWideLdsMax) {
+      a.wide = 1;
+      a.rs = (Idx)rsw;
+      a.fpl = (Idx)(op.R * rsw);
+      a.ncw = (Idx)(rsw / 16);
+      a.ld_per = a.st_per = a.R * a.ncw;
+      a.cpr = 0;
+      vec = false;
+    }
   } else {
     vec = V > 1 && one_word && op.C % V == 0 && base % 16 == 0;
     a.rs = (Idx)(op.C * op.us);
@@ -780,7 +1015,16 @@ void prepare_lds(const Op& op, char* data, const LaunchCfg& cfg, Args<Idx>& a, P
     a.ld_per = a.C * a.wu;
     a.vpr = vec ? (Idx)(op.C / V) : 1;
     a.st_per = vec ? a.vpr : a.C * a.wu;
+    if (wide_ok(op) && op.C * op.U + 32 <= kWideLdsMax) {
+      a.wide = 1;
+      a.cpr = 0;
+      vec = false;
+    }
   }
+  a.hi = op.base + op.bytes();
+  a.fU.init(op.kind == OP_COL ? op.us : op.U);  // the wide paths' unit: slice or unit
+  a.fCU.init(op.C * op.U);
+  a.fncw.init(a.ncw ? a.ncw : 1);
   a.cps_log2 = -1;
   if (a.cps && (a.cps & (a.cps - 1)) == 0) {
     a.cps_log2 = 0;
@@ -792,8 +1036,10 @@ void prepare_lds(const Op& op, char* data, const LaunchCfg& cfg, Args<Idx>& a, P
   a.fst.init(a.st_per);
   a.fvpr.init(a.vpr);
   p.K = pick_K((int64_t)a.nfp, (int64_t)a.fpl, cfg.lds_target);
+  if (a.wide && op.kind == OP_ROW)  // the run of K rows and its two partial chunks
+    while (p.K > 1 && p.K * (int64_t)a.fpl + 32 > kWideLdsMax) --p.K;
   a.K = (Idx)p.K;
-  p.lds = p.K * (int64_t)a.fpl;
+  p.lds = p.K * (int64_t)a.fpl + (a.wide && op.kind == OP_ROW ? 32 : 0);
   p.threads = p.lds > 96 * 1024 ? cfg.threads_big : cfg.threads;
   p.groups = ((int64_t)a.nfp + p.K - 1) / p.K;
   p.variant = km * 10 + wl * 2 + (vec ? 1 : 0);
@@ -1274,17 +1520,25 @@ __global__ __launch_bounds__(256) void k_cycle_pipe(CycArgs a) {
   }
 }
 
-// Owner of a device allocation on device `dev`: its deleter waits for that
-// device to go idle before freeing, so a pass enqueued earlier never reads
-// freed memory.  The caller's current device is restored.
-std::shared_ptr<void> device_owned(void* p, int dev) {
-  return std::shared_ptr<void>(p, [dev](void* q) {
+// The counter that table uploads on this thread are charged to.
+thread_local std::atomic<int64_t>* t_account = nullptr;
+
+// Owner of a device allocation of `bytes` on device `dev`: its deleter
+// waits for that device to go idle before freeing, so a pass enqueued
+// earlier never reads freed memory.  The caller's current device is
+// restored.  The bytes are charged to this thread's table account, if any,
+// until freed.
+std::shared_ptr<void> device_owned(void* p, int dev, int64_t bytes) {
+  std::atomic<int64_t>* acct = t_account;
+  if (acct) acct->fetch_add(bytes, std::memory_order_relaxed);
+  return std::shared_ptr<void>(p, [dev, bytes, acct](void* q) {
     int cur = 0;
     const bool have = hipGetDevice(&cur) == hipSuccess;
     if (have && cur != dev) (void)hipSetDevice(dev);
     (void)hipDeviceSynchronize();
     (void)hipFree(q);
     if (have && cur != dev) (void)hipSetDevice(cur);
+    if (acct) acct->fetch_sub(bytes, std::memory_order_relaxed);
   });
 }
 
@@ -1298,7 +1552,7 @@ const void* cycle_table_device(const CycleTable& t, hipError_t* err) {
   const size_t sb = t.segs.size() * sizeof(CycleSeg), lb = t.slots.size() * sizeof(uint32_t);
   void* p = nullptr;
   if ((*err = hipMalloc(&p, std::max<size_t>(sb + lb, 16))) != hipSuccess) return nullptr;
-  std::shared_ptr<void> own = device_owned(p, dev);
+  std::shared_ptr<void> own = device_owned(p, dev, (int64_t)std::max<size_t>(sb + lb, 16));
   if (sb && (*err = hipMemcpy(p, t.segs.data(), sb, hipMemcpyHostToDevice)) != hipSuccess)
     return nullptr;
   if (lb && (*err = hipMemcpy((char*)p + sb, t.slots.data(), lb, hipMemcpyHostToDevice)) !=
@@ -1397,6 +1651,236 @@ hipError_t prepare_cycle_args(const Op& op, char* data, char* scratch, int64_t s
   return hipSuccess;
 }
 
+// ---------------------------------------------------------------------------
+// Cycle following of cells that are not whole aligned 16-byte words
+// (k_cycle_u).  An item owns the cell-relative bytes [r0, r0 + n) of every
+// cell of its segment, r0 = g*G with G = (lanes - 2) * 16, so items that
+// share cells never touch each other's bytes.  At each position the item's
+// lanes cover the aligned 16-byte chunks of the destination's range (at
+// most lanes - 1 of them): lane l takes chunk X = (first chunk) + 16*l.
+// Its bytes come from the source range at the same cell-relative offsets:
+// the aligned source chunk at or below X + (src - dst) and the one after
+// it, funnel-shifted by their misalignment (the same for the whole item).
+// Every lane loads one source chunk and takes the next one from the lane
+// above it with a wave shuffle (ds_bpermute), so each source byte is
+// requested once.  Bytes read outside the range are never stored.  A
+// destination chunk wholly inside the range is stored with one 16-byte
+// store, the range's end chunks as aligned pieces of 8, 4, 2 and 1 bytes.
+// Source chunks that would reach outside the pass's bytes (or the boundary
+// area) are gathered byte by byte.  The step order and hazards are those
+// of k_cycle: within a wave, every load of a batch completes before any of
+// its stores.
+// ---------------------------------------------------------------------------
+struct CycUArgs {
+  uint64_t G;         // cell bytes per item
+  uint64_t groups;    // items per (matrix, segment)
+  uint32_t lanes_log2;
+  FD64 fgroups;
+};
+
+// dword j (0..7) of the 32 bytes a:b
+__device__ __forceinline__ uint32_t pick8(const W16& a, const W16& b, uint32_t j) {
+  const uint32_t x0 = (j & 1) ? a.y : a.x, x1 = (j & 1) ? a.w : a.z;
+  const uint32_t x2 = (j & 1) ? b.y : b.x, x3 = (j & 1) ? b.w : b.z;
+  const uint32_t y0 = (j & 2) ? x1 : x0, y1 = (j & 2) ? x3 : x2;
+  return (j & 4) ? y1 : y0;
+}
+
+// bytes sig .. sig+15 of the 32 bytes a:b
+__device__ __forceinline__ W16 funnel16(const W16& a, const W16& b, uint32_t sig) {
+  const uint32_t q = sig >> 2, r = sig & 3;
+  W16 o;
+  o.x = __builtin_amdgcn_alignbyte(pick8(a, b, q + 1), pick8(a, b, q), r);
+  o.y = __builtin_amdgcn_alignbyte(pick8(a, b, q + 2), pick8(a, b, q + 1), r);
+  o.z = __builtin_amdgcn_alignbyte(pick8(a, b, q + 3), pick8(a, b, q + 2), r);
+  o.w = __builtin_amdgcn_alignbyte(pick8(a, b, q + 4), pick8(a, b, q + 3), r);
+  return o;
+}
+
+// 16 bytes from offset x of base, only those within [lo, hi) read (the
+// others are zero)
+__device__ __forceinline__ W16 gather16(const char* base, int64_t x, int64_t lo, int64_t hi) {
+  uint32_t d[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < 16; ++j)
+    if (x + j >= lo && x + j < hi)
+      d[j >> 2] |= (uint32_t)(unsigned char)base[x + j] << (8 * (j & 3));
+  W16 o;
+  o.x = d[0];
+  o.y = d[1];
+  o.z = d[2];
+  o.w = d[3];
+  return o;
+}
+
+// the next lane's chunk within groups of 2^ll lanes (the group's last lane
+// gets its own back)
+__device__ __forceinline__ W16 shfl_next(const W16& v, int width) {
+  W16 o;
+  o.x = __shfl_down(v.x, 1, width);
+  o.y = __shfl_down(v.y, 1, width);
+  o.z = __shfl_down(v.z, 1, width);
+  o.w = __shfl_down(v.w, 1, width);
+  return o;
+}
+
+// Stores the bytes of chunk v (at offset X from data) that lie in
+// [x0, x1), x0 < x1 within the chunk: 16 bytes at once when that is the
+// whole chunk, else as aligned pieces of 8, 4, 2 and 1 bytes.
+__device__ __forceinline__ void cycu_store(char* data, int64_t X, int64_t x0, int64_t x1,
+                                           const W16& v) {
+  if (x0 == X && x1 == X + 16) {
+    *reinterpret_cast<W16*>(data + X) = v;
+    return;
+  }
+  const uint64_t lo = (uint64_t)v.x | ((uint64_t)v.y << 32), hi = (uint64_t)v.z | ((uint64_t)v.w << 32);
+  const uintptr_t d0 = (uintptr_t)data;
+  for (int64_t x = x0; x < x1;) {
+    const uint32_t o = (uint32_t)(x - X);  // byte of the chunk
+    const uint64_t bits = (o < 8 ? lo >> (8 * o) : hi >> (8 * (o - 8)));
+    const uintptr_t at = d0 + (uint64_t)x;
+    if ((at & 7) == 0 && x + 8 <= x1) {
+      *reinterpret_cast<uint64_t*>(data + x) = bits;
+      x += 8;
+    } else if ((at & 3) == 0 && x + 4 <= x1) {
+      *reinterpret_cast<uint32_t*>(data + x) = (uint32_t)bits;
+      x += 4;
+    } else if ((at & 1) == 0 && x + 2 <= x1) {
+      *reinterpret_cast<uint16_t*>(data + x) = (uint16_t)bits;
+      x += 2;
+    } else {
+      data[x] = (char)bits;
+      x += 1;
+    }
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_cycle_u(CycArgs a, CycUArgs u) {
+  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int L = 1 << u.lanes_log2;
+  const uint32_t lane = (uint32_t)(gtid & (uint64_t)(L - 1));
+  const uint64_t item = gtid >> u.lanes_log2;
+  // every lane of a live item takes part in the shuffles: an item is whole
+  // lanes of one wave (L divides 64), and items past the end exit together
+  if (item >= a.items) return;
+  const uint64_t rest = u.fgroups.div(item);
+  const uint64_t g = item - rest * u.groups;
+  const uint64_t b = a.fnseg.div(rest);
+  const uint64_t sidx = rest - b * a.nseg;
+  const int64_t r0 = (int64_t)(g * u.G);
+  const int64_t n = (int64_t)a.U - r0 < (int64_t)u.G ? (int64_t)a.U - r0 : (int64_t)u.G;
+  const int64_t m = (int64_t)(a.base + b * a.RCU);  // the matrix, offset from data
+  const int64_t lo = (int64_t)a.base, hi = (int64_t)(a.base + a.B * a.RCU);
+  const int64_t blo = 0, bhi = (int64_t)(a.B * a.nslots * a.U);
+  const int64_t U = (int64_t)a.U;
+  const uintptr_t d0 = (uintptr_t)a.data;
+  // lane's destination chunk of the range that starts at offset r (from data)
+  auto chunk = [&](int64_t r) { return r - (int64_t)((d0 + (uint64_t)r) & 15) + 16 * (int64_t)lane; };
+  // the source chunk this lane loads for destination range dr taking the
+  // range at src of sbase (valid bytes [slo, shi)); *xs: the source offset
+  // of the lane's destination chunk, *sig: its misalignment
+  auto load = [&](int64_t dr, const char* sbase, int64_t src, int64_t slo, int64_t shi, int64_t* xs,
+                  uint32_t* sig) -> W16 {
+    *xs = chunk(dr) - dr + src;
+    *sig = (uint32_t)(((uintptr_t)sbase + (uint64_t)*xs) & 15);
+    const int64_t s0 = *xs - *sig;
+    W16 v = {0, 0, 0, 0};
+    if (s0 >= slo && s0 + 16 <= shi) v = *reinterpret_cast<const W16*>(sbase + s0);
+    return v;
+  };
+  // the 16 bytes of the lane's destination chunk from its loaded chunk c
+  // (all lanes of the item call this together: it shuffles)
+  auto assemble = [&](const W16& c, int64_t xs, uint32_t sig, const char* sbase, int64_t slo,
+                      int64_t shi) -> W16 {
+    const W16 nx = shfl_next(c, L);
+    const int64_t s0 = xs - sig;
+    if (s0 >= slo && s0 + 32 <= shi) return sig ? funnel16(c, nx, sig) : c;
+    return gather16(sbase, xs, slo, shi);
+  };
+  const CycleSeg sg = a.segs[sidx];
+  uint32_t p = sg.start;
+  // a whole cycle's last write takes the old start cell: read it first
+  W16 fin = {0, 0, 0, 0};
+  if (sg.src_slot < 0) {
+    const int64_t dl = m + (int64_t)sg.last * U + r0;
+    int64_t xs;
+    uint32_t sig;
+    const W16 c = load(dl, a.data, m + (int64_t)p * U + r0, lo, hi, &xs, &sig);
+    fin = assemble(c, xs, sig, a.data, lo, hi);
+  }
+  const char* bnd = a.boundary;
+  for (uint32_t t = 0; t < sg.len; t += D) {
+    uint32_t q[D];
+    uint32_t pp = p;
+#pragma unroll
+    for (int k = 0; k < D; ++k) q[k] = pp = cyc_src(a, pp);
+    W16 v[D];
+    int64_t xs[D];
+    uint32_t sig[D];
+    // the batch's loads (the last write of a cut segment reads its saved cell)
+    uint32_t dst = p;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      if (t + k < sg.len && !(t + k + 1 == sg.len && sg.src_slot < 0)) {
+        const int64_t dr = m + (int64_t)dst * U + r0;
+        if (t + k + 1 < sg.len)
+          v[k] = load(dr, a.data, m + (int64_t)q[k] * U + r0, lo, hi, &xs[k], &sig[k]);
+        else
+          v[k] = load(dr, bnd, (int64_t)((b * a.nslots + (uint64_t)sg.src_slot) * a.U) + r0, blo,
+                      bhi, &xs[k], &sig[k]);
+      }
+      dst = q[k];
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      if (t + k < sg.len) {
+        if (t + k + 1 < sg.len)
+          v[k] = assemble(v[k], xs[k], sig[k], a.data, lo, hi);
+        else if (sg.src_slot < 0)
+          v[k] = fin;
+        else
+          v[k] = assemble(v[k], xs[k], sig[k], bnd, blo, bhi);
+      }
+    }
+    dst = p;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      if (t + k < sg.len) {
+        const int64_t dr = m + (int64_t)dst * U + r0;
+        const int64_t X = chunk(dr);
+        const int64_t x0 = X < dr ? dr : X, x1 = X + 16 < dr + n ? X + 16 : dr + n;
+        if (x0 < x1) cycu_store(a.data, X, x0, x1, v[k]);
+      }
+      dst = q[k];
+    }
+    p = q[D - 1];
+  }
+}
+
+template <typename Word>
+hipError_t launch_cycle_u(CycArgs a, hipStream_t stream) {
+  if (a.nslots) {
+    const uint64_t words = a.B * a.nslots * (a.U / sizeof(Word));
+    const uint64_t blocks = std::min<uint64_t>((words + 255) / 256, 256ull * 64);
+    hipLaunchKernelGGL((k_cycle_save<Word>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
+  }
+  CycUArgs u;
+  // lanes per item: G = (lanes - 2) * 16 cell bytes need at most lanes - 1
+  // destination chunks, so the last lane only hands its chunk down
+  uint32_t ll = 2;
+  while (ll < 6 && ((uint64_t)((1u << ll) - 2) * 16) < a.U) ++ll;
+  u.lanes_log2 = ll;
+  u.G = (uint64_t)((1u << ll) - 2) * 16;
+  u.groups = (a.U + u.G - 1) / u.G;
+  u.fgroups.init(u.groups);
+  a.items = a.B * a.nseg * u.groups;
+  const uint64_t blocks = ((a.items << ll) + 255) / 256;
+  if (blocks >= (1ull << 31)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_cycle_u<8>), dim3((uint32_t)blocks), dim3(256), 0, stream, a, u);
+  return hipGetLastError();
+}
+
 hipError_t launch_cycle(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
                         int variant, hipStream_t stream, const void* table) {
   CycArgs a;
@@ -1405,6 +1889,14 @@ hipError_t launch_cycle(const Op& op, char* data, char* scratch, int64_t scratch
   hipError_t e =
       prepare_cycle_args(op, data, scratch, scratch_bytes, &variant, a, &w, &empty, table);
   if (e != hipSuccess || empty) return e;
+  if (w < 16 && variant == 0) {  // cells that are not whole 16-byte words
+    switch (w) {
+      case 8: return launch_cycle_u<uint64_t>(a, stream);
+      case 4: return launch_cycle_u<uint32_t>(a, stream);
+      case 2: return launch_cycle_u<uint16_t>(a, stream);
+      default: return launch_cycle_u<uint8_t>(a, stream);
+    }
+  }
   switch (w) {
     case 16: return launch_cycle_w<W16>(a, variant, stream);
     case 8: return launch_cycle_w<uint64_t>(a, variant, stream);
@@ -1499,31 +1991,53 @@ __device__ __forceinline__ void block_body(const BlkArgs& a, const uint64_t f0, 
   }
   __syncthreads();
   char* out = a.data + a.base + f0 * a.F;
-  if (V > 1 && (nf * a.n) % V == 0 && ((uintptr_t)out & 15) == 0) {
-    // one word per cell, V consecutive output cells of the run per 16-byte
-    // store (they may belong to two blocks when n is not a multiple of V)
-    const uint32_t total = nf * a.n / V;
-    const bool inner = a.ndig && a.frad[0].d % V == 0 && a.n % V == 0;
+  if (V > 1 && ((uint64_t)nf * a.F) % 16 == 0 && ((uintptr_t)out & 15) == 0) {
+    // 16-byte stores: chunk t of the run is words t*V .. t*V+V-1 (cells of
+    // one or several words, possibly of two blocks).  The first word's
+    // block, cell and source come from divisions; each following word's from
+    // the previous one's: the next word of the cell, or the next cell, whose
+    // source is one step of the mixed-radix odometer over the cell's output
+    // digits (or the next table entry).
+    const uint32_t per = a.n * a.wpc;  // words per block
+    const uint32_t total = (uint32_t)((uint64_t)nf * a.F / 16);
     for (uint32_t t = tid; t < total; t += nt) {
-      const uint32_t o0 = t * V;
+      const uint32_t w0 = t * V;
+      uint32_t k = a.fnw.div(w0);
+      const uint32_t r = w0 - k * per;
+      uint32_t o = a.wpc == 1 ? r : a.fwpc.div(r);
+      uint32_t x = r - o * a.wpc;
+      DigitOdometer od;
+      if (a.ndig) od.init(a.frad, a.stride, a.ndig, o);
+      else od.src = blk_src(a, o);
       Packed<Word, V> tmp;
-      if (inner) {
-        // the V cells lie in one block and differ in its fastest output digit
-        const uint32_t k = a.fn.div(o0);
-        const uint32_t s0 = blk_src(a, o0 - k * a.n);
+      if (a.wpc == 1 && a.ndig && od.dg[0] + V <= a.frad[0].d) {
+        // the V one-word cells differ only in their fastest digit
 #pragma unroll
         for (int v = 0; v < V; ++v)
           tmp.e[v] = *reinterpret_cast<const Word*>(
-              lds + k * a.Fl + lds_off((s0 + v * a.stride[0]) * w, a.pad));
-      } else {
+              lds + k * a.Fl + lds_off((od.src + v * a.stride[0]) * w, a.pad));
+        *reinterpret_cast<typename Packed<Word, V>::P*>(out + (uint64_t)t * 16) = tmp.p;
+        continue;
+      }
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-          const uint32_t o = o0 + v, k = a.fn.div(o);
-          tmp.e[v] = *reinterpret_cast<const Word*>(lds + k * a.Fl +
-                                                  lds_off(blk_src(a, o - k * a.n) * w, a.pad));
+      for (int v = 0; v < V; ++v) {
+        tmp.e[v] = *reinterpret_cast<const Word*>(lds + k * a.Fl +
+                                                lds_off((od.src * a.wpc + x) * w, a.pad));
+        if (++x == a.wpc) {
+          x = 0;
+          if (++o == a.n) {  // the next block's first cell
+            o = 0;
+            ++k;
+            if (a.ndig) od.init(a.frad, a.stride, a.ndig, 0);
+            else od.src = blk_src(a, 0);
+          } else if (a.ndig) {
+            od.step(a.frad, a.stride, a.ndig);
+          } else {
+            od.src = blk_src(a, o);
+          }
         }
       }
-      *reinterpret_cast<typename Packed<Word, V>::P*>(out + (uint64_t)o0 * w) = tmp.p;
+      *reinterpret_cast<typename Packed<Word, V>::P*>(out + (uint64_t)t * 16) = tmp.p;
     }
   } else {
     const uint32_t per = a.n * a.wpc;
@@ -1559,7 +2073,7 @@ const void* block_table_device(const BlockTable& t, hipError_t* err) {
   const size_t nb = t.src.size() * (i16 ? 2 : 4);
   void* p = nullptr;
   if ((*err = hipMalloc(&p, std::max<size_t>(nb, 16))) != hipSuccess) return nullptr;
-  std::shared_ptr<void> own = device_owned(p, dev);
+  std::shared_ptr<void> own = device_owned(p, dev, (int64_t)std::max<size_t>(nb, 16));
   if ((*err = hipMemcpy(p, i16 ? (const void*)h16.data() : (const void*)t.src.data(), nb,
                         hipMemcpyHostToDevice)) != hipSuccess)
     return nullptr;
@@ -1610,7 +2124,7 @@ hipError_t prepare_block_args(const Op& op, char* data, const LaunchCfg& cfg, Bl
   a.vec16 = base % 16 == 0 ? 1 : 0;
   a.idx16 = t.cells <= 65536 ? 1 : 0;
   const int64_t V = w < 16 ? 16 / w : 1;
-  const int vec = V > 1 && a.wpc == 1 && a.vec16;
+  const int vec = V > 1 && a.vec16;
   a.chunks = (uint32_t)op.R;
   // blocks whose size is not a multiple of 16 bytes lie in LDS back to back
   // (16-byte loads may then straddle two blocks); otherwise padding keeps
@@ -1626,7 +2140,6 @@ hipError_t prepare_block_args(const Op& op, char* data, const LaunchCfg& cfg, Bl
   // a multiple of the blocks that make 16-byte runs and whole 16-byte stores
   // (past the LDS target when that many blocks fit the workgroup's LDS)
   int64_t mK = 16 / gcd64(F, 16);
-  if (vec) mK = mK / gcd64(mK, V / gcd64(op.R, V)) * (V / gcd64(op.R, V));
   if (K < mK && mK <= op.B && mK * (int64_t)a.Fl <= 160 * 1024) K = mK;
   if (K >= mK) K -= K % mK;
   a.K = (uint32_t)K;
@@ -2473,5 +2986,8 @@ hipError_t configure_kernels(int64_t bytes) {
   cfg_word<uint8_t>(bytes, e);
   return e;
 }
+
+TableAccount::TableAccount(std::atomic<int64_t>* c) : prev(t_account) { t_account = c; }
+TableAccount::~TableAccount() { t_account = prev; }
 
 }  // namespace eithot

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <atomic>
 #include <memory>
 #include <utility>
 #include <vector>
@@ -37,9 +38,19 @@ hipError_t launch_op(const Op& op, char* data, char* scratch, int64_t scratch_by
 // capturing the op into a graph.
 hipError_t prepare_op(const Op& op);
 
-// Owner of device memory `p` on device `dev` whose deleter waits for the
-// device to go idle before hipFree.
-std::shared_ptr<void> device_owned(void* p, int dev);
+// While in scope, pass tables uploaded by this thread are charged to `*c`
+// (added on upload, subtracted when freed); the counter must outlive them.
+struct TableAccount {
+  explicit TableAccount(std::atomic<int64_t>* c);
+  ~TableAccount();
+  TableAccount(const TableAccount&) = delete;
+  TableAccount& operator=(const TableAccount&) = delete;
+  std::atomic<int64_t>* prev;
+};
+// Owner of device memory `p` (`bytes`) on device `dev` whose deleter waits
+// for the device to go idle before hipFree; the bytes stay charged to the
+// thread's TableAccount at upload time until then.
+std::shared_ptr<void> device_owned(void* p, int dev, int64_t bytes);
 
 // Bytes of LDS the op's LDS kernel uses per workgroup with cfg (0 if staged).
 int64_t op_lds_bytes(const Op& op, const LaunchCfg& cfg);

@@ -95,6 +95,37 @@ struct FD32 {
   EH_HD uint32_t mod(uint32_t x) const { return x - div(x) * d; }
 };
 
+// Source cell of a digit permutation, stepped cell by cell.  A cell index
+// o has mixed-radix digits d_k (radix frad[k].d, fastest first) and its
+// source is sum d_k * stride[k]; step() moves to o + 1 with one add per
+// digit that changes (an odometer), instead of ndig divisions.
+struct DigitOdometer {
+  uint32_t dg[8];
+  uint32_t src;
+  EH_HD void init(const FD32* frad, const uint32_t* stride, uint32_t ndig, uint32_t o) {
+    src = 0;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      dg[d] = 0;
+      if (d >= (int)ndig) continue;
+      const uint32_t nq = frad[d].div(o);
+      dg[d] = o - nq * frad[d].d;
+      src += dg[d] * stride[d];
+      o = nq;
+    }
+  }
+  EH_HD void step(const FD32* frad, const uint32_t* stride, uint32_t ndig) {
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      if (d >= (int)ndig) break;
+      src += stride[d];
+      if (++dg[d] < frad[d].d) break;
+      dg[d] = 0;
+      src -= frad[d].d * stride[d];
+    }
+  }
+};
+
 template <typename Idx>
 struct FDsel;
 template <>

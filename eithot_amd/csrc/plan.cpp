@@ -320,7 +320,11 @@ void walk_parallel(const Src& src, uint64_t N, int64_t lmax, std::vector<uint64_
     }
     if (total == 1) continue;  // a fixed point
     if (total <= L) {
-      t.segs.push_back({all[i0].start, (uint32_t)total, -1, 0});
+      // a whole cycle records its last position (the end of its last piece)
+      const Piece& lp = all[chain.back()];
+      uint64_t last = lp.start;
+      for (uint32_t x = 1; x < lp.len; ++x) last = src(last);
+      t.segs.push_back({all[i0].start, (uint32_t)total, -1, (uint32_t)last});
       continue;
     }
     // segment starts every lmax positions along the cycle from all[i0].start,
@@ -386,7 +390,7 @@ std::shared_ptr<CycleTable> build_cycle_table_maps(const std::vector<CycleMap>& 
   for (uint64_t p0 = 0; p0 < (uint64_t)N; ++p0) {
     if (seen[p0 >> 6] >> (p0 & 63) & 1) continue;
     starts.clear();
-    uint64_t k = 0, q = p0, left = 0;
+    uint64_t k = 0, q = p0, left = 0, last = p0;
     do {
       if (left == 0) {
         starts.push_back((uint32_t)q);
@@ -394,12 +398,13 @@ std::shared_ptr<CycleTable> build_cycle_table_maps(const std::vector<CycleMap>& 
       }
       --left;
       seen[q >> 6] |= 1ull << (q & 63);
+      last = q;
       q = src(q);
       ++k;
     } while (q != p0);
     if (k == 1) continue;
     if (starts.size() == 1) {
-      t->segs.push_back({(uint32_t)p0, (uint32_t)k, -1, 0});
+      t->segs.push_back({(uint32_t)p0, (uint32_t)k, -1, (uint32_t)last});
       continue;
     }
     const uint32_t n = (uint32_t)starts.size(), first_slot = (uint32_t)t->slots.size();
@@ -645,14 +650,18 @@ constexpr uint64_t kCycleMaxCells = 1ull << 26;
 constexpr uint64_t kCycleMaxCellsRegroup = 1ull << 28;
 constexpr double kCostCycle = 1.15;  // 4.6-5.8 TB/s for cells of 128 B to 384 KB, 16-byte words
 constexpr double kCostBlock = 1.1;   // fused block permutation
-// Cells whose size is not a multiple of 16 bytes move in narrower words
-// (0.8-0.9 TB/s measured for 2- and 4-byte words).
+// Cells whose size is not a multiple of 16 bytes: the single-launch kernel
+// moves them in 16-byte chunks with a funnel shift (kCostCycleShift); the
+// batched kernel moves narrower words (0.8-0.9 TB/s measured for 2- and
+// 4-byte words).
+constexpr double kCostCycleShift = 1.35;
 thread_local bool g_cycle_word_cost = true;  // off: every cell width costs the same
 thread_local bool g_cycle_cell_pref = false;  // on: larger cells cost less
+thread_local bool g_cycle_shift = true;       // narrow cells move in shifted 16-byte chunks
 inline double cycle_cost(int64_t U) {
   const int64_t w = pow2_factor(U, 16);
   if (!g_cycle_word_cost) return kCostCycle;
-  if (w < 16) return w == 8 ? 2.0 : w == 4 ? 4.0 : 6.0;
+  if (w < 16) return g_cycle_shift ? kCostCycleShift : w == 8 ? 2.0 : w == 4 ? 4.0 : 6.0;
   if (!g_cycle_cell_pref) return kCostCycle;
   return U >= 512 ? kCostCycle - 0.1 : U >= 256 ? kCostCycle - 0.05 : kCostCycle;
 }
@@ -905,6 +914,7 @@ static int64_t* tuning_field(Tuning& t, const char* key) {
   if (!strcmp(key, "lds_threads_auto")) return &t.lds_threads_auto;
   if (!strcmp(key, "segtile_nt")) return &t.segtile_nt;
   if (!strcmp(key, "lower_mask")) return &t.lower_mask;
+  if (!strcmp(key, "hold_floor")) return &t.hold_floor;
   return nullptr;
 }
 
@@ -1731,6 +1741,7 @@ static int make_plan_search(const int64_t* dims_in, const int* perm1, int rank, 
                             Plan& plan) {
   g_cycle_word_cost = prof.cycle_word_cost != 0;
   g_cycle_cell_pref = prof.cycle_cell_pref != 0;
+  g_cycle_shift = prof.cycle_shift != 0;
   // Search: decomposer order (pre/post, or the one asked for) x how hard
   // staged passes are avoided; cycle passes that later compose with their
   // neighbours make a split lowering cheaper than a staged one, which only

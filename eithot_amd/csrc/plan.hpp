@@ -83,7 +83,7 @@ struct CycleSeg {
   uint32_t len;      // positions written: start, src(start), ...
   int32_t src_slot;  // slot holding the old cell the last write takes; -1: the
                      // segment is a whole cycle and takes old(start)
-  uint32_t pad;
+  uint32_t last;     // whole cycle: its last position (the one src maps to start)
 };
 // One cell map of a cycle pass: cells in blocks of bc cells; a block is an
 // R x C matrix of m-cell units, transposed: cell q = (j*R + i)*m + x of a
@@ -195,6 +195,8 @@ struct DeviceProfile {
   int64_t cycle_lmax = 0;         // > 0: cycle segments of at most this many positions
   int64_t cycle_cell_pref = 0;    // 1: cycle passes of larger cells cost less
   int64_t lower_mask = LOWER_ALL; // lowering options the planner may use
+  int cycle_shift = 1;            // cycle passes of narrow cells run the shifted 16-byte
+                                  // kernel (0: word by word, as in batched launches)
   double staged_mult = 1.0;       // planner-internal: weight of staged passes in the search
 };
 
@@ -221,6 +223,7 @@ struct Tuning {
   int64_t lds_threads_auto = 1;
   int64_t segtile_nt = 3;
   int64_t lower_mask = LOWER_ALL;
+  int64_t hold_floor = 64ll << 20;  // device bytes the library may keep whatever alpha allows
   uint64_t epoch = 0;  // bumped on every change (plan caches key on it)
 };
 Tuning tuning_get();

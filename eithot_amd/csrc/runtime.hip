@@ -136,27 +136,23 @@ PlanPtr Device::plan_for(const int64_t* dims, const int* perm1, int rank, size_t
   return out;
 }
 
-int64_t Device::held_bytes() {
-  std::lock_guard<std::mutex> lk(mu);
-  int64_t n = arena_bytes;
-  plans.for_each([&](const std::string&, const PlanPtr& p) { n += p->device_table_bytes(id); });
-  return n;
-}
+int64_t Device::held_bytes() { return arena_bytes + cached_tables.load(std::memory_order_relaxed); }
 
 void Device::trim_locked(int64_t budget, const Plan* keep) {
+  // what this call will hold: the resident tables, those of `keep` still to
+  // be uploaded, and the arena it ends with (the current one, or one grown
+  // to its need); the plans are walked only when that exceeds the budget
+  const int64_t need = keep ? keep->scratch_bytes : 0;
+  const int64_t keep_res = keep ? keep->device_table_bytes(id) : 0;
+  const int64_t keep_tables = keep ? std::max(keep->table_bytes, keep_res) : 0;
+  int64_t others = cached_tables.load(std::memory_order_relaxed) - keep_res;
+  auto held_after = [&] { return std::max(arena_bytes, need) + keep_tables + others; };
+  if (held_after() <= budget) return;
   std::vector<PlanPtr> order;  // most recently used first
   {
     std::lock_guard<std::mutex> lk(mu);
     plans.for_each([&](const std::string&, const PlanPtr& p) { order.push_back(p); });
   }
-  int64_t keep_tables = keep ? std::max(keep->table_bytes, keep->device_table_bytes(id)) : 0;
-  int64_t others = 0;
-  for (const PlanPtr& p : order)
-    if (p.get() != keep) others += p->device_table_bytes(id);
-  // the arena this call ends with: the current one, or one grown to its need
-  const int64_t need = keep ? keep->scratch_bytes : 0;
-  auto held_after = [&] { return std::max(arena_bytes, need) + keep_tables + others; };
-  if (held_after() <= budget) return;
   // an arena larger than this call needs is given back first
   if (arena && arena_bytes > need) {
     if (arena_ready) (void)hipEventSynchronize(arena_ready);
@@ -297,17 +293,21 @@ static int transpose_impl(void* dev, const int64_t* dims, const int* perm, int o
   }
   {
     std::lock_guard<std::mutex> lk(d->enqueue_mu);
-    // the library's device bytes stay within alpha of this tensor (or this
-    // plan's own need, when one footprint exceeds it)
+    // the library's device bytes stay within alpha of this tensor, or this
+    // plan's own need when one footprint exceeds it, or the holding floor
+    // (which spares calls on small tensors from evicting and re-uploading
+    // the tables of the others)
     const int64_t own = plan->scratch_bytes + plan->table_bytes;
-    const int64_t budget =
-        std::max<int64_t>((int64_t)(alpha * (double)plan->tensor_bytes), own);
+    const int64_t budget = std::max<int64_t>(
+        std::max<int64_t>((int64_t)(alpha * (double)plan->tensor_bytes), own),
+        tuning_get().hold_floor);
     d->trim_locked(budget, plan.get());
     if (plan->scratch_bytes > 0) {
       rc = d->reserve_arena_locked(plan->scratch_bytes, stream);
       if (rc) return rc;
     }
     std::vector<float>* times = g_timing ? &g_pass_ms : nullptr;
+    TableAccount acct(&d->cached_tables);
     rc = run_plan(d, *plan, (char*)dev, d->arena, plan->scratch_bytes, stream, times);
     if (rc) return rc;
     if (plan->scratch_bytes > 0) {

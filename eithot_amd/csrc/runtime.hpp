@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -33,9 +34,12 @@ struct Device {
   char* arena = nullptr;             // scratch for staged passes and cycle boundaries
   int64_t arena_bytes = 0;
   hipEvent_t arena_ready = nullptr;  // recorded after the last pass that used the arena
+  // device bytes of the pass tables uploaded for plans of this cache (and
+  // not yet freed; an evicted plan a request still holds counts until freed)
+  std::atomic<int64_t> cached_tables{0};
 
   // plans cached per device, and the host bytes of their pass tables
-  static constexpr size_t kPlanCacheEntries = 1024;
+  static constexpr size_t kPlanCacheEntries = 8192;
   static constexpr uint64_t kPlanCacheHostBytes = 16ull << 30;
 
   // The cached plan for these arguments, planned on first sight (outside

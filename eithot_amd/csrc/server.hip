@@ -159,6 +159,7 @@ struct eithot_server {
       if (rc == EITHOT_OK) {
         // launches are exclusive with other threads' table eviction
         std::lock_guard<std::mutex> lk(dev->enqueue_mu);
+        TableAccount acct(&dev->cached_tables);
         rc = run_plan(dev, *r->plan, r->dev, need > 0 ? pool_mem + off : nullptr, need, st,
                       nullptr);
       }

@@ -86,8 +86,10 @@ int eithot_inplace_transpose(void* data, int source, int rank, const int* dim,
  * plus every pass table resident on the device for the plans it caches (this
  * call's and earlier ones').  Each call first frees other plans' tables,
  * least recently used first, and shrinks an arena larger than it needs, so
- * that this stays within alpha * tensor bytes, or within the call's own
- * need when one pass footprint exceeds that.  0 when nothing is held.
+ * that this stays within the largest of alpha * tensor bytes, the call's
+ * own need (when one pass footprint exceeds that) and the "hold_floor"
+ * tuning value (64 MiB, which spares calls on small tensors from evicting
+ * and re-uploading each other's tables).  0 when nothing is held.
  * Server pools and batch scratch belong to their handles and are not
  * counted. */
 size_t eithot_last_peak_extra_bytes(void);
@@ -222,6 +224,9 @@ int eithot_plan_segments(const int64_t* dims, const int* perm, int order, size_t
  *   "lds_threads_auto" tile, column, row and block workgroups sized for one round of
  *                    8 loads per thread (1) or from "threads"/"threads_big" (0)
  *   "xcd_remap"      keep consecutive footprints on one XCD (1)
+ *   "hold_floor"     device bytes (arena and pass tables) the library may keep
+ *                    after a call even when alpha * tensor bytes is smaller
+ *                    (64 MiB; 0: alpha alone bounds them)
  *   "lower_mask"     lowering options of each 2-D transpose (31 = all): 1 LDS
  *                    tile, 2 Catanzaro column/row passes, 4 hierarchical split,
  *                    8 tile through scratch, 16 cycle following; a transpose

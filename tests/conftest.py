@@ -20,3 +20,17 @@ def torch_cuda():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     return torch
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _strict_extra_memory():
+    """Extra device memory is checked against alpha itself: no holding floor
+    (test_extra_memory_follows_alpha covers the default floor)."""
+    try:
+        import eithot_amd as E
+        old = E.set_tuning("hold_floor", 0)
+    except Exception:  # the library is not built: nothing to configure
+        yield
+        return
+    yield
+    E.set_tuning("hold_floor", old)

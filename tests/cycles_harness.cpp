@@ -168,12 +168,15 @@ int ch_check(int64_t R, int64_t C, int64_t lmax) {
   auto src = [&](uint64_t p) { return (p % R) * C + p / R; };
   for (const CycleSeg& sg : tab->segs) {
     if ((int64_t)sg.len > lmax) return 1;
-    uint64_t p = sg.start;
+    uint64_t p = sg.start, prev = p;
     for (uint32_t t = 0; t < sg.len; ++t) {
       if (p >= N) return 2;
       ++hits[p];
+      prev = p;
       p = src(p);
     }
+    // a whole cycle records its last position, which maps back to its start
+    if (sg.src_slot < 0 && (sg.last != prev || src(sg.last) != sg.start)) return 4;
   }
   for (uint64_t p = 0; p < N; ++p) {
     const bool fixed = src(p) == p;

@@ -127,10 +127,12 @@ def test_beyond_half_hbm_172GB(torch_cuda):
 
 def test_extra_memory_follows_alpha(torch_cuda):
     # tables and arena held for a large tensor are given back when a small
-    # tensor's call has a smaller budget
+    # tensor's call has a smaller budget (with no holding floor), and kept
+    # within the floor otherwise
     import eithot_amd as E
     torch = torch_cuda
     _free(torch)
+    floor = E.set_tuning("hold_floor", 0)
     big = [32, 48, 64, 40, 36, 8]
     vol = int(np.prod(big))
     buf = torch.empty(vol * 4, dtype=torch.uint8, device="cuda:0")
@@ -150,6 +152,16 @@ def test_extra_memory_follows_alpha(torch_cuda):
     # the large tensor's plan uploads its tables again and stays correct
     E.transpose_tensor(buf, [8, 40, 48, 36, 32, 64], [5, 3, 6, 2, 4, 1], 4, 1, 0.05)
     assert E.verify_pattern(buf, big, [1, 2, 3, 4, 5, 6], 4, 1, 1) == 0
+    # with the default floor of 64 MiB the small call keeps the large
+    # plan's tables
+    E.set_tuning("hold_floor", 64 << 20)
+    try:
+        E.transpose_tensor(t, [128, 36, 8, 5], [1, 3, 2, 4], 4, 1, 0.05)
+        torch.cuda.synchronize()
+        assert 0.05 * svol * 4 < E.last_peak_extra_bytes() <= 64 << 20
+        assert_bitexact(t.cpu().numpy(), host, "small tensor, back")
+    finally:
+        E.set_tuning("hold_floor", floor)
     del buf
     _free(torch)
 

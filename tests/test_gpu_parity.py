@@ -168,7 +168,7 @@ REGROUP_CASES = [([15, 24, 20, 13, 25], [5, 3, 1, 2, 4], 4), ([30, 9, 2, 66, 63]
                  ([10, 37, 6, 30, 27], [5, 4, 3, 2, 1], 1),
                  ([12, 15, 11, 11, 13, 10], [2, 5, 3, 6, 4, 1], 8),
                  ([35, 27, 5, 3, 3, 8], [6, 4, 2, 5, 1, 3], 4),
-                 ([2, 4, 103, 21, 117], [5, 3, 2, 4, 1], 8),
+                 ([2, 8, 103, 21, 112], [5, 3, 2, 4, 1], 8),
                  ([6, 2, 30, 25, 17, 18], [3, 6, 5, 1, 4, 2], 2),
                  ([32, 48, 64, 5, 6, 8], [6, 4, 2, 5, 1, 3], 4)]
 
@@ -242,3 +242,81 @@ def test_segtile_plans(torch_cuda):
         want = host.copy()
         want[elem:] = O.permute(np.ascontiguousarray(host[elem:]), dims, [p - 1 for p in perm], elem)
         assert_bitexact(got, want, f"segtile offset {dims} {perm} elem {elem}")
+
+
+NARROW_CATANZARO = [([3, 300, 257], [1, 3, 2], 1), ([3, 257, 300], [1, 3, 2], 1),
+                    ([9, 41, 1074], [1, 3, 2], 2), ([13, 445, 332], [1, 3, 2], 2),
+                    ([18, 200, 131], [1, 3, 2], 1), ([5, 512, 100], [1, 3, 2], 2),
+                    ([7, 1000, 999], [1, 3, 2], 1), ([6, 64, 96], [1, 3, 2], 4)]
+
+
+def test_narrow_unit_catanzaro_passes(torch_cuda):
+    # column and row passes over units that are not 16-byte words (3 to 26
+    # bytes; panels that do not divide the grid): rows move as aligned
+    # 16-byte chunks through LDS; at the tensor's own address and one
+    # element past it (every row then starts at another offset mod 16)
+    import eithot_amd as E
+    from oracle import pyoracle as O
+    old = {k: E.set_tuning(k, v) for k, v in (("lower_mask", 2), ("regroup", 0))}
+    try:
+        for dims, perm, elem in NARROW_CATANZARO:
+            ops, _ = E.plan_describe(dims, perm, elem, 1, 0.05)
+            assert {o["kind"] for o in ops} == {2, 3}, (dims, ops)
+            got, want, _ = run_case(torch_cuda, dims, perm, elem, seed=16, alpha=0.05)
+            assert_bitexact(got, want, f"narrow {dims} {perm} elem {elem}")
+            vol = int(np.prod(dims))
+            host = O.fill(vol + 1, elem, 17, 1)
+            dev = torch_cuda.from_numpy(host.copy()).to("cuda:0")
+            print(f"case offset 1 dims={dims} perm={perm} elem={elem}", flush=True)
+            E.transpose(dev, 1, len(dims), dims, [p - 1 for p in perm], elem, 1, 0.05)
+            torch_cuda.cuda.synchronize()
+            want = host.copy()
+            want[elem:] = O.permute(np.ascontiguousarray(host[elem:]), dims, [p - 1 for p in perm],
+                                    elem)
+            assert_bitexact(dev.cpu().numpy(), want, f"narrow offset {dims} {perm} elem {elem}")
+    finally:
+        for k, v in old.items():
+            E.set_tuning(k, v)
+
+
+SHIFTED_CYCLE_CASES = [([129, 60, 70], [1, 3, 2], 1),      # 129-byte cells, 1-byte words
+                       ([257, 40, 30], [1, 3, 2], 2),      # 514-byte cells
+                       ([300, 40, 17, 2], [1, 3, 2, 4], 2),  # 600-byte cells, 8-byte words
+                       ([1273, 6, 35], [1, 3, 2], 4),      # 5092-byte cells: 6 item groups each
+                       ([2547, 35, 6], [1, 3, 2], 2),
+                       ([3, 14, 2, 15, 58, 20], [5, 2, 6, 1, 3, 4], 4)]  # composed maps
+
+
+def test_shifted_cycle_passes(torch_cuda):
+    # cycle following of cells that are not whole 16-byte words, moved in
+    # aligned 16-byte chunks with a funnel shift: whole and cut cycles
+    # (segments of 3 positions), at the tensor's address and one element past
+    # it; every kernel variant other than the default moves such cells word
+    # by word and is run once for comparison
+    import eithot_amd as E
+    from oracle import pyoracle as O
+    regroup = E.set_tuning("regroup", 0)
+    try:
+        for lmax in (0, 3):
+            E.set_tuning("cycle_lmax", lmax)
+            for variant in (0, 4):
+                E.set_tuning("cycle_variant", variant)
+                for dims, perm, elem in SHIFTED_CYCLE_CASES:
+                    ops, _ = E.plan_describe(dims, perm, elem, 1, 0.5)
+                    assert any(o["kind"] == 4 and o["U"] % 16 for o in ops), (dims, ops)
+                    got, want, _ = run_case(torch_cuda, dims, perm, elem, seed=18, alpha=0.5)
+                    assert_bitexact(got, want, f"lmax {lmax} variant {variant} {dims} {perm}")
+                    vol = int(np.prod(dims))
+                    host = O.fill(vol + 1, elem, 19, 1)
+                    dev = torch_cuda.from_numpy(host.copy()).to("cuda:0")
+                    print(f"case offset 1 dims={dims} perm={perm} elem={elem}", flush=True)
+                    E.transpose(dev, 1, len(dims), dims, [p - 1 for p in perm], elem, 1, 0.5)
+                    torch_cuda.cuda.synchronize()
+                    want = host.copy()
+                    want[elem:] = O.permute(np.ascontiguousarray(host[elem:]), dims,
+                                            [p - 1 for p in perm], elem)
+                    assert_bitexact(dev.cpu().numpy(), want, f"offset lmax {lmax} {dims} {perm}")
+    finally:
+        E.set_tuning("cycle_lmax", 0)
+        E.set_tuning("cycle_variant", 0)
+        E.set_tuning("regroup", regroup)

@@ -35,6 +35,9 @@ def mh():
     L.mh_div64.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
     L.mh_div32.restype = ctypes.c_uint32
     L.mh_div32.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+    L.mh_odometer.restype = ctypes.c_int64
+    L.mh_odometer.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                              ctypes.c_int]
     return L
 
 
@@ -85,3 +88,21 @@ def test_magic_division(mh):
         for x in xs:
             x &= (1 << 64) - 1
             assert mh.mh_div64(d, x) == x // d, (d, x)
+
+
+def test_digit_odometer(mh):
+    # the fused block pass steps a cell's source digit by digit: every step
+    # equals the direct digit sum, for radices and strides of digit
+    # permutations (nested transposes) and arbitrary ones
+    import numpy as np
+    rng = np.random.default_rng(11)
+    cases = [([16], [1]), ([3, 5], [5, 1]), ([2, 2, 2, 2, 2, 2, 2, 2], [128, 64, 32, 16, 8, 4, 2, 1]),
+             ([7, 1, 9], [9, 63, 1]), ([126], [1])]
+    for _ in range(40):
+        nd = int(rng.integers(1, 9))
+        rad = [int(x) for x in rng.integers(1, 6, size=nd)]
+        cases.append((rad, [int(x) for x in rng.integers(0, 1000, size=nd)]))
+    for rad, st in cases:
+        r = (ctypes.c_uint32 * 8)(*rad)
+        t = (ctypes.c_uint32 * 8)(*st)
+        assert mh.mh_odometer(r, t, len(rad)) == -1, (rad, st)
