@@ -167,11 +167,106 @@ __device__ __forceinline__ Idx row_src(const Args<Idx>& a, Idx i, Idx j) {
 // tile pass: K matrices (or matrix slices) per workgroup
 // LDS: footprint k row i at k*fpl + i*rs (rs padded), cell j at +j*us
 // ---------------------------------------------------------------------------
+// Loads the contiguous run [g0, g1) (offsets from data) into lds at its
+// address modulo 16: lds byte y holds byte a0 + y, a0 = g0 rounded down to
+// an aligned 16-byte chunk, so aligned chunks land on aligned LDS chunks.
+// A chunk wholly inside the pass's bytes [base, hi) moves with one 16-byte
+// load (bytes outside the run are loaded but never stored); a chunk at the
+// pass's edge moves word by word.
+template <typename Word, typename Idx>
+__device__ __forceinline__ void run_to_lds(const Args<Idx>& a, int64_t g0, int64_t g1, int64_t a0,
+                                           unsigned char* lds) {
+  constexpr Idx w = sizeof(Word);
+  const Idx tid = threadIdx.x, nt = blockDim.x;
+  const Idx n = (Idx)((g1 - a0 + 15) / 16);
+  for (Idx t0 = tid; t0 < n; t0 += nt * UNR) {
+    W16 v[UNR];
+    bool full[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const Idx t = t0 + (Idx)u * nt;
+      full[u] = false;
+      if (t >= n) continue;
+      const int64_t cs = a0 + (int64_t)t * 16;
+      if (cs >= (int64_t)a.base && cs + 16 <= (int64_t)a.hi) {
+        v[u] = *reinterpret_cast<const W16*>(a.data + cs);
+        full[u] = true;
+      } else {
+        const int64_t b0 = cs < g0 ? g0 : cs, b1 = cs + 16 < g1 ? cs + 16 : g1;
+        for (int64_t b = b0; b < b1; b += w)
+          *reinterpret_cast<Word*>(lds + (b - a0)) = *reinterpret_cast<const Word*>(a.data + b);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u)
+      if (full[u]) *reinterpret_cast<W16*>(lds + (t0 + (Idx)u * nt) * 16) = v[u];
+  }
+}
+
+// Wide tile pass (whole units of 1- and 2-byte words, S == 1): the
+// workgroup's nf matrices are one contiguous run, loaded as aligned 16-byte
+// chunks (run_to_lds); every cell is then copied word by word to its
+// transposed place in a second LDS copy of the run, which goes back to HBM
+// as aligned 16-byte stores (word by word in the run's two edge chunks).
+// Cells are visited in input order (ord 0: consecutive threads read
+// consecutive cells) or output order (ord 1), whichever the host found to
+// have fewer LDS bank conflicts.
+template <typename Word, typename Idx>
+__device__ __forceinline__ void tile_wide(const Args<Idx>& a, const Idx f0, const Idx nf,
+                                          unsigned char* lds) {
+  constexpr Idx w = sizeof(Word);
+  const Idx tid = threadIdx.x, nt = blockDim.x;
+  const uintptr_t d0 = (uintptr_t)a.data;
+  const int64_t g0 = (int64_t)(a.base + (uint64_t)f0 * a.RCU);
+  const int64_t g1 = g0 + (int64_t)nf * (int64_t)a.RCU;
+  const int64_t a0 = g0 - (int64_t)((d0 + (uint64_t)g0) & 15);
+  const Idx sh = (Idx)(g0 - a0);
+  unsigned char* out = lds + (((Idx)a.K * (Idx)a.RCU + 32 + 15) & ~(Idx)15);
+  run_to_lds<Word, Idx>(a, g0, g1, a0, lds);
+  __syncthreads();
+  const Idx RC = a.R * a.C, U = (Idx)a.U, RCU = (Idx)a.RCU;
+  const Idx ncell = nf * RC;
+  for (Idx q = tid; q < ncell; q += nt) {
+    const Idx k = a.fld.div(q);  // ld_per == R*C
+    const Idx r = q - k * RC;
+    Idx i, j;
+    if (a.wide == 1) {  // r = i*C + j: input order
+      i = a.fC.div(r);
+      j = r - i * a.C;
+    } else {            // r = j*R + i: output order
+      j = a.fR.div(r);
+      i = r - j * a.R;
+    }
+    const Idx s = sh + k * RCU + (i * a.C + j) * U;
+    const Idx d = sh + k * RCU + (j * a.R + i) * U;
+    for (Idx x = 0; x < U; x += w)
+      *reinterpret_cast<Word*>(out + d + x) = *reinterpret_cast<const Word*>(lds + s + x);
+  }
+  __syncthreads();
+  const Idx n = (Idx)((g1 - a0 + 15) / 16);
+  for (Idx t = tid; t < n; t += nt) {
+    const int64_t cs = a0 + (int64_t)t * 16;
+    if (cs >= g0 && cs + 16 <= g1) {
+      *reinterpret_cast<W16*>(a.data + cs) = *reinterpret_cast<const W16*>(out + t * 16);
+    } else {
+      const int64_t b0 = cs < g0 ? g0 : cs, b1 = cs + 16 < g1 ? cs + 16 : g1;
+      for (int64_t b = b0; b < b1; b += w)
+        *reinterpret_cast<Word*>(a.data + b) = *reinterpret_cast<const Word*>(out + (b - a0));
+    }
+  }
+}
+
 template <typename Word, typename Idx, int VEC>
 __device__ __forceinline__ void tile_body(const Args<Idx>& a, const Idx f0, unsigned char* lds) {
   const Idx nf = (a.nfp - f0) < a.K ? (a.nfp - f0) : a.K;
   const Idx tid = threadIdx.x, nt = blockDim.x;
   constexpr Idx w = sizeof(Word);
+  if constexpr (sizeof(Word) < 16) {
+    if (a.wide) {
+      tile_wide<Word, Idx>(a, f0, nf, lds);
+      return;
+    }
+  }
 
   if (a.cpr) {
     // nf whole contiguous matrices: rows of C*us bytes in 16-byte chunks;
@@ -769,6 +864,58 @@ __global__ __launch_bounds__(256) void k_stage_in(Args<Idx> a) {
   }
 }
 
+// Staged row pass of whole units (S == 1), part 2: the G rows of the group
+// lie in scratch as one copy of their bytes; row fp0 + fl is gathered back
+// through the row map, one aligned 16-byte output chunk per thread (V words,
+// the units stepped one by one, not divided per word) and one 16-byte store;
+// the row's two edge chunks are stored word by word.  grid.y strides rows.
+template <typename Word, typename Idx, int MAP>
+__global__ __launch_bounds__(256) void k_stage_in_row(Args<Idx> a) {
+  constexpr Idx w = sizeof(Word);
+  constexpr int V = 16 / sizeof(Word) > 1 ? 16 / sizeof(Word) : 1;
+  const uintptr_t d0 = (uintptr_t)a.data;
+  const int64_t CU = (int64_t)a.CU;
+  const Idx U = (Idx)a.U;
+  for (uint64_t fl = blockIdx.y; fl < a.G; fl += gridDim.y) {
+    const Foot<Idx, OP_ROW> f = decode<Idx, OP_ROW>(a, (Idx)(a.fp0 + fl));
+    const int64_t r0 = (int64_t)f.addr, r1 = r0 + CU;
+    const int64_t a0 = r0 - (int64_t)((d0 + (uint64_t)r0) & 15);
+    const uint64_t nch = (uint64_t)((r1 - a0 + 15) / 16);
+    const char* sc = a.scratch + fl * (uint64_t)CU;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nch;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+      const int64_t cs = a0 + 16 * (int64_t)t;
+      const int64_t b0 = cs < r0 ? r0 : cs, b1 = cs + 16 < r1 ? cs + 16 : r1;
+      const Idx x = (Idx)(b0 - r0);
+      Idx q = a.fU.div(x);
+      Idx xu = x - q * U;
+      Idx src = row_src<Idx, MAP>(a, f.row, q);
+      auto next = [&]() {
+        xu += w;
+        if (xu == U) {
+          xu = 0;
+          if (++q < a.C) src = row_src<Idx, MAP>(a, f.row, q);
+        }
+      };
+      if (V > 1 && b0 == cs && b1 == cs + 16) {
+        Packed<Word, V> tmp;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          tmp.e[v] = *reinterpret_cast<const Word*>(sc + (uint64_t)src * U + xu);
+          next();
+        }
+        *reinterpret_cast<typename Packed<Word, V>::P*>(a.data + cs) = tmp.p;
+      } else {
+        for (int64_t b = b0; b < b1; b += w) {
+          *reinterpret_cast<Word*>(a.data + b) =
+              *reinterpret_cast<const Word*>(sc + (uint64_t)src * U + xu);
+          next();
+        }
+      }
+    }
+  }
+}
+
 // staged tile, part 2: out-of-place LDS-tiled transpose of the footprints
 // held in scratch (each an R x C row-major matrix of us-byte cells) back into
 // the tensor, where output cell (j, i) of footprint f lives at
@@ -887,6 +1034,7 @@ Args<Idx> make_args(const Op& op, char* data, int64_t wbytes) {
   a.fW.init(a.W);
   a.fP.init(a.P);
   a.fwu.init(a.wu);
+  a.fU.init(op.kind == OP_COL ? op.us : op.U);  // wide and staged-row paths: slice or unit
   if (op.kind != OP_TILE) a.g.init(op.m, op.n, op.c, op.a, op.b, op.k, op.binv);
   return a;
 }
@@ -954,6 +1102,30 @@ inline bool wide_ok(const Op& op) {
   return !(aligned && op.W % V == 0 && op.C % op.W == 0);
 }
 
+// Tile passes of whole units in 1- or 2-byte words take the wide path
+// (tile_wide), unless one-word cells already pack into 16-byte stores over
+// rows of whole 16-byte chunks.  As for column and row passes the choice
+// depends on the op only.
+inline bool tile_wide_ok(const Op& op) {
+  if (op.staged || op.kind != OP_TILE || op.us != op.U) return false;
+  const int64_t w = word_bytes(op, 0);
+  if (w > 2) return false;
+  const bool packs = op.U == w && op.R % (16 / w) == 0 && op.base % 16 == 0 && (op.C * op.U) % 16 == 0;
+  return !packs;
+}
+
+// LDS bytes of one copy of a wide tile's run of K matrices of F bytes (the
+// run plus the partial chunks at its two ends), and of both copies
+inline int64_t tile_wide_copy(int64_t K, int64_t F) { return (K * F + 32 + 15) / 16 * 16; }
+
+// Distinct 4-byte LDS banks (of 64) that 64 lanes hit with accesses `stride`
+// bytes apart.
+inline int lds_banks_hit(int64_t stride) {
+  uint64_t seen = 0;
+  for (int64_t l = 0; l < 64; ++l) seen |= 1ull << ((l * stride / 4) % 64);
+  return __builtin_popcountll(seen);
+}
+
 // Fills every LDS-kernel field of `a` for op (not staged) under cfg.
 template <typename Idx>
 void prepare_lds(const Op& op, char* data, const LaunchCfg& cfg, Args<Idx>& a, Prepared& p) {
@@ -979,6 +1151,18 @@ void prepare_lds(const Op& op, char* data, const LaunchCfg& cfg, Args<Idx>& a, P
     a.ld_per = a.cps ? a.R * a.C * a.cps : a.R * a.C * a.wu;
     a.vpr = vec ? (Idx)(op.R / V) : 1;
     a.st_per = vec ? (Idx)(op.C * a.vpr) : a.R * a.C * a.wu;
+    const int64_t F = op.R * op.C * op.U;
+    if (tile_wide_ok(op) && 2 * tile_wide_copy(1, F) <= kWideLdsMax) {
+      // input order unless output order spreads the copies over more banks
+      // (input order: consecutive threads write R*U bytes apart; output
+      // order: they read C*U bytes apart)
+      a.wide = lds_banks_hit(op.C * op.U) > lds_banks_hit(op.R * op.U) ? 2 : 1;
+      a.cpr = 0;
+      a.cps = 0;
+      vec = false;
+      a.fpl = (Idx)F;
+      a.ld_per = a.st_per = a.R * a.C;
+    }
   } else if (op.kind == OP_COL) {
     vec = V > 1 && one_word && op.W % V == 0 && op.C % op.W == 0 && base % 16 == 0 &&
           (op.C * op.U) % 16 == 0;
@@ -1035,11 +1219,15 @@ void prepare_lds(const Op& op, char* data, const LaunchCfg& cfg, Args<Idx>& a, P
   a.fld.init(a.ld_per);
   a.fst.init(a.st_per);
   a.fvpr.init(a.vpr);
-  p.K = pick_K((int64_t)a.nfp, (int64_t)a.fpl, cfg.lds_target);
+  p.K = pick_K((int64_t)a.nfp, (a.wide && op.kind == OP_TILE ? 2 : 1) * (int64_t)a.fpl,
+               cfg.lds_target);
   if (a.wide && op.kind == OP_ROW)  // the run of K rows and its two partial chunks
     while (p.K > 1 && p.K * (int64_t)a.fpl + 32 > kWideLdsMax) --p.K;
+  if (a.wide && op.kind == OP_TILE)  // two copies of the run of K matrices
+    while (p.K > 1 && 2 * tile_wide_copy(p.K, a.fpl) > kWideLdsMax) --p.K;
   a.K = (Idx)p.K;
   p.lds = p.K * (int64_t)a.fpl + (a.wide && op.kind == OP_ROW ? 32 : 0);
+  if (a.wide && op.kind == OP_TILE) p.lds = 2 * tile_wide_copy(p.K, a.fpl);
   p.threads = p.lds > 96 * 1024 ? cfg.threads_big : cfg.threads;
   p.groups = ((int64_t)a.nfp + p.K - 1) / p.K;
   p.variant = km * 10 + wl * 2 + (vec ? 1 : 0);
@@ -1129,9 +1317,10 @@ hipError_t launch_staged(Args<Idx> a, const Op& op, char* scratch, int64_t scrat
     a.G = std::min<uint64_t>(G, nfp - f0);
     const uint64_t words = a.G * (uint64_t)op.footprint_units() * a.wu;
     const uint64_t blocks = std::min<uint64_t>((words + 255) / 256, 256ull * 64);
-    if (KIND == OP_TILE && a.S == 1) {
-      // whole matrices: one contiguous range
-      hipError_t e = hipMemcpyAsync(scratch, a.data + a.base + f0 * a.RCU, a.G * a.RCU,
+    if ((KIND == OP_TILE || KIND == OP_ROW) && a.S == 1) {
+      // whole matrices, or whole rows: one contiguous range
+      const uint64_t fpb_all = KIND == OP_TILE ? a.RCU : a.CU;
+      hipError_t e = hipMemcpyAsync(scratch, a.data + a.base + f0 * fpb_all, a.G * fpb_all,
                                     hipMemcpyDeviceToDevice, st);
       if (e != hipSuccess) return e;
     } else {
@@ -1168,6 +1357,12 @@ hipError_t launch_staged(Args<Idx> a, const Op& op, char* scratch, int64_t scrat
         hipLaunchKernelGGL((k_tile_oop<Word, Idx, V>), dim3(gx, gy), dim3(256), lds, st, a, tg);
       else
         hipLaunchKernelGGL((k_tile_oop<Word, Idx, 1>), dim3(gx, gy), dim3(256), lds, st, a, tg);
+    } else if (KIND == OP_ROW && a.S == 1) {
+      const uint64_t nch = (a.CU + 31) / 16;  // a row's 16-byte chunks, edges included
+      const uint32_t gx = (uint32_t)std::min<uint64_t>((nch + 255) / 256, 1024);
+      const uint32_t gy = (uint32_t)std::min<uint64_t>(a.G, 65535);
+      hipLaunchKernelGGL((k_stage_in_row<Word, Idx, (KIND == OP_ROW ? MAP : MAP_ROWG)>), dim3(gx, gy),
+                         dim3(256), 0, st, a);
     } else {
       hipLaunchKernelGGL((k_stage_in<Word, Idx, KIND, MAP>), dim3((uint32_t)blocks), dim3(256), 0, st,
                          a);
@@ -1678,23 +1873,22 @@ struct CycUArgs {
   FD64 fgroups;
 };
 
-// dword j (0..7) of the 32 bytes a:b
-__device__ __forceinline__ uint32_t pick8(const W16& a, const W16& b, uint32_t j) {
-  const uint32_t x0 = (j & 1) ? a.y : a.x, x1 = (j & 1) ? a.w : a.z;
-  const uint32_t x2 = (j & 1) ? b.y : b.x, x3 = (j & 1) ? b.w : b.z;
-  const uint32_t y0 = (j & 2) ? x1 : x0, y1 = (j & 2) ? x3 : x2;
-  return (j & 4) ? y1 : y0;
-}
-
-// bytes sig .. sig+15 of the 32 bytes a:b
+// bytes sig .. sig+15 of the 32 bytes a:b: dwords q .. q+4 (q = sig / 4) of
+// a:b are picked by a shift of 2 dwords and then of 1 (two levels of
+// selects), then funnel-shifted by sig % 4 bytes
 __device__ __forceinline__ W16 funnel16(const W16& a, const W16& b, uint32_t sig) {
-  const uint32_t q = sig >> 2, r = sig & 3;
-  W16 o;
-  o.x = __builtin_amdgcn_alignbyte(pick8(a, b, q + 1), pick8(a, b, q), r);
-  o.y = __builtin_amdgcn_alignbyte(pick8(a, b, q + 2), pick8(a, b, q + 1), r);
-  o.z = __builtin_amdgcn_alignbyte(pick8(a, b, q + 3), pick8(a, b, q + 2), r);
-  o.w = __builtin_amdgcn_alignbyte(pick8(a, b, q + 4), pick8(a, b, q + 3), r);
-  return o;
+  const uint32_t r = sig & 3;
+  const bool h = (sig & 8) != 0, o = (sig & 4) != 0;
+  const uint32_t e0 = h ? a.z : a.x, e1 = h ? a.w : a.y, e2 = h ? b.x : a.z;
+  const uint32_t e3 = h ? b.y : a.w, e4 = h ? b.z : b.x, e5 = h ? b.w : b.y;
+  const uint32_t f0 = o ? e1 : e0, f1 = o ? e2 : e1, f2 = o ? e3 : e2;
+  const uint32_t f3 = o ? e4 : e3, f4 = o ? e5 : e4;
+  W16 out;
+  out.x = __builtin_amdgcn_alignbyte(f1, f0, r);
+  out.y = __builtin_amdgcn_alignbyte(f2, f1, r);
+  out.z = __builtin_amdgcn_alignbyte(f3, f2, r);
+  out.w = __builtin_amdgcn_alignbyte(f4, f3, r);
+  return out;
 }
 
 // 16 bytes from offset x of base, only those within [lo, hi) read (the
@@ -1779,23 +1973,29 @@ __global__ __launch_bounds__(256) void k_cycle_u(CycArgs a, CycUArgs u) {
   auto chunk = [&](int64_t r) { return r - (int64_t)((d0 + (uint64_t)r) & 15) + 16 * (int64_t)lane; };
   // the source chunk this lane loads for destination range dr taking the
   // range at src of sbase (valid bytes [slo, shi)); *xs: the source offset
-  // of the lane's destination chunk, *sig: its misalignment
+  // of the lane's destination chunk, *sig: its misalignment.  A lane's chunk
+  // feeds its own destination chunk and the one of the lane below it, so
+  // only lanes whose destination chunk starts before dr + n + 16 load.
   auto load = [&](int64_t dr, const char* sbase, int64_t src, int64_t slo, int64_t shi, int64_t* xs,
                   uint32_t* sig) -> W16 {
-    *xs = chunk(dr) - dr + src;
+    const int64_t X = chunk(dr);
+    *xs = X - dr + src;
     *sig = (uint32_t)(((uintptr_t)sbase + (uint64_t)*xs) & 15);
     const int64_t s0 = *xs - *sig;
     W16 v = {0, 0, 0, 0};
-    if (s0 >= slo && s0 + 16 <= shi) v = *reinterpret_cast<const W16*>(sbase + s0);
+    if (X < dr + n + 16 && s0 >= slo && s0 + 16 <= shi) v = *reinterpret_cast<const W16*>(sbase + s0);
     return v;
   };
   // the 16 bytes of the lane's destination chunk from its loaded chunk c
-  // (all lanes of the item call this together: it shuffles)
-  auto assemble = [&](const W16& c, int64_t xs, uint32_t sig, const char* sbase, int64_t slo,
-                      int64_t shi) -> W16 {
+  // (all lanes of the item call this together: it shuffles); a lane whose
+  // destination chunk starts at or past dr + n stores nothing and gathers
+  // nothing
+  auto assemble = [&](const W16& c, int64_t dr, int64_t xs, uint32_t sig, const char* sbase,
+                      int64_t slo, int64_t shi) -> W16 {
     const W16 nx = shfl_next(c, L);
     const int64_t s0 = xs - sig;
     if (s0 >= slo && s0 + 32 <= shi) return sig ? funnel16(c, nx, sig) : c;
+    if (chunk(dr) >= dr + n) return c;
     return gather16(sbase, xs, slo, shi);
   };
   const CycleSeg sg = a.segs[sidx];
@@ -1807,7 +2007,7 @@ __global__ __launch_bounds__(256) void k_cycle_u(CycArgs a, CycUArgs u) {
     int64_t xs;
     uint32_t sig;
     const W16 c = load(dl, a.data, m + (int64_t)p * U + r0, lo, hi, &xs, &sig);
-    fin = assemble(c, xs, sig, a.data, lo, hi);
+    fin = assemble(c, dl, xs, sig, a.data, lo, hi);
   }
   const char* bnd = a.boundary;
   for (uint32_t t = 0; t < sg.len; t += D) {
@@ -1832,16 +2032,19 @@ __global__ __launch_bounds__(256) void k_cycle_u(CycArgs a, CycUArgs u) {
       }
       dst = q[k];
     }
+    dst = p;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       if (t + k < sg.len) {
+        const int64_t dr = m + (int64_t)dst * U + r0;
         if (t + k + 1 < sg.len)
-          v[k] = assemble(v[k], xs[k], sig[k], a.data, lo, hi);
+          v[k] = assemble(v[k], dr, xs[k], sig[k], a.data, lo, hi);
         else if (sg.src_slot < 0)
           v[k] = fin;
         else
-          v[k] = assemble(v[k], xs[k], sig[k], bnd, blo, bhi);
+          v[k] = assemble(v[k], dr, xs[k], sig[k], bnd, blo, bhi);
       }
+      dst = q[k];
     }
     dst = p;
 #pragma unroll
@@ -1871,8 +2074,11 @@ hipError_t launch_cycle_u(CycArgs a, hipStream_t stream) {
   uint32_t ll = 2;
   while (ll < 6 && ((uint64_t)((1u << ll) - 2) * 16) < a.U) ++ll;
   u.lanes_log2 = ll;
-  u.G = (uint64_t)((1u << ll) - 2) * 16;
-  u.groups = (a.U + u.G - 1) / u.G;
+  const uint64_t gmax = (uint64_t)((1u << ll) - 2) * 16;
+  // cells larger than that are cut into equal groups (the last one may be
+  // shorter by less than a group count of bytes)
+  u.groups = (a.U + gmax - 1) / gmax;
+  u.G = (a.U + u.groups - 1) / u.groups;
   u.fgroups.init(u.groups);
   a.items = a.B * a.nseg * u.groups;
   const uint64_t blocks = ((a.items << ll) + 255) / 256;
@@ -1921,8 +2127,12 @@ struct BlkArgs {
   FD32 fnw, fwpc, fch, fF16, fFw;             // n * wpc, wpc, chunks, F / 16, F / w
   uint32_t ndig, stride[8];                   // digit form of the table (ndig = 0: table)
   FD32 frad[8];
-  uint32_t dense;  // 1: the K blocks lie in LDS back to back (Fl == F, no padding)
+  uint32_t dense;  // 1: the K blocks lie in LDS back to back, padded as one run
   FD32 fn;         // n
+  // pattern stores (one word per cell): the run's 16-byte chunks repeat
+  // their source offsets every P chunks (P * 16 = lcm(F, 16) bytes, a
+  // multiple of the pad interval); 0: off
+  uint32_t P;
 };
 
 // source cell of output cell o: digit form, or the gather table
@@ -1947,16 +2157,165 @@ __device__ __forceinline__ uint32_t lds_off(uint32_t x, uint32_t pad) {
   return pad ? x + ((x >> pad) << 2) : x;
 }
 
+// LDS byte offset of byte x of block k of the workgroup when blocks are
+// padded one by one at Fl bytes apart (not dense)
+__device__ __forceinline__ uint32_t blk_lds(const BlkArgs& a, uint32_t k, uint32_t x) {
+  return k * a.Fl + lds_off(x, a.pad);
+}
+
+// Dense layout (blocks whose size is not a multiple of 16 bytes, or a
+// tensor address that is not 16-byte aligned): the workgroup's nf blocks are
+// one run [g0, g1) (offsets from data) kept in LDS at its address modulo 16
+// and padded as one byte range: LDS byte lds_off(y) holds byte a0 + y, a0 =
+// g0 rounded down to an aligned 16-byte chunk, sh = g0 - a0.  Chunks wholly
+// inside the pass's bytes load with one 16-byte load, those at its edge word
+// by word.  Output chunks wholly inside the run are gathered V words at a
+// time and stored with one 16-byte store (with pattern offsets when P
+// threads divide the workgroup, else through the digit odometer); the run's
+// two edge chunks are stored word by word.
+template <typename Word, int V>
+__device__ __forceinline__ void block_run(const BlkArgs& a, const uint64_t f0, const uint32_t nf,
+                                          unsigned char* lds) {
+  constexpr uint32_t w = sizeof(Word);
+  const uint32_t tid = threadIdx.x, nt = blockDim.x;
+  const uintptr_t d0 = (uintptr_t)a.data;
+  const int64_t g0 = (int64_t)(a.base + f0 * a.F), g1 = g0 + (int64_t)nf * (int64_t)a.F;
+  const int64_t a0 = g0 - (int64_t)((d0 + (uint64_t)g0) & 15);
+  const uint32_t sh = (uint32_t)(g0 - a0);
+  const int64_t lo = (int64_t)a.base, hi = (int64_t)(a.base + a.B * a.F);
+  const uint32_t n16 = (uint32_t)((g1 - a0 + 15) / 16);
+  for (uint32_t t0 = tid; t0 < n16; t0 += nt * UNR) {
+    W16 v[UNR];
+    bool full[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const uint32_t t = t0 + u * nt;
+      full[u] = false;
+      if (t >= n16) continue;
+      const int64_t cs = a0 + 16 * (int64_t)t;
+      if (cs >= lo && cs + 16 <= hi) {
+        v[u] = *reinterpret_cast<const W16*>(a.data + cs);
+        full[u] = true;
+      } else {
+        const int64_t b0 = cs < g0 ? g0 : cs, b1 = cs + 16 < g1 ? cs + 16 : g1;
+        for (int64_t b = b0; b < b1; b += w)
+          *reinterpret_cast<Word*>(lds + lds_off((uint32_t)(b - a0), a.pad)) =
+              *reinterpret_cast<const Word*>(a.data + b);
+      }
+    }
+    // a 16-byte chunk never straddles a pad (intervals of at least 64 bytes)
+#pragma unroll
+    for (int u = 0; u < UNR; ++u)
+      if (full[u]) lds_put16(lds + lds_off((t0 + u * nt) * 16, a.pad), v[u]);
+  }
+  __syncthreads();
+  const uint32_t per = a.n * a.wpc;  // words per block
+  // the word of run byte b (b in [g0, g1), a multiple of w)
+  auto word_at = [&](int64_t b) -> Word {
+    const uint32_t wi = (uint32_t)((b - g0) / w);
+    const uint32_t k = a.fnw.div(wi);
+    const uint32_t r = wi - k * per;
+    const uint32_t o = a.wpc == 1 ? r : a.fwpc.div(r);
+    const uint32_t x = r - o * a.wpc;
+    const uint32_t src = blk_src(a, o);
+    return *reinterpret_cast<const Word*>(
+        lds + lds_off(sh + k * (uint32_t)a.F + (src * a.wpc + x) * w, a.pad));
+  };
+  auto edge = [&](int64_t cs) {
+    const int64_t b0 = cs < g0 ? g0 : cs, b1 = cs + 16 < g1 ? cs + 16 : g1;
+    for (int64_t b = b0; b < b1; b += w) *reinterpret_cast<Word*>(a.data + b) = word_at(b);
+  };
+  if (V > 1 && a.P && nt % a.P == 0) {
+    // one word per cell: run byte 16t - sh of chunk t lies at rel within
+    // its period of P16 bytes, the same rel for every chunk of the thread
+    const uint32_t P16 = a.P * 16;
+    const uint32_t pos = tid % a.P;
+    const uint32_t rel = (16 * pos + P16 - sh) % P16;
+    uint32_t off[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const uint32_t c = rel / w + (uint32_t)v;  // cell from the period's start
+      const uint32_t kb = a.fn.div(c);
+      off[v] = lds_off(sh + kb * (uint32_t)a.F + blk_src(a, c - kb * a.n) * w, a.pad);
+    }
+    // period of chunk t (-1 for the run's first chunk when it is partial)
+    int32_t pi = (int32_t)(tid / a.P) - ((pos == 0 && sh > 0) ? 1 : 0);
+    const int32_t q = (int32_t)(nt / a.P);
+    for (uint32_t t = tid; t < n16; t += nt, pi += q) {
+      const int64_t cs = a0 + 16 * (int64_t)t;
+      if (cs >= g0 && cs + 16 <= g1) {
+        const uint32_t base = lds_off((uint32_t)pi * P16, a.pad);
+        Packed<Word, V> tmp;
+#pragma unroll
+        for (int v = 0; v < V; ++v) tmp.e[v] = *reinterpret_cast<const Word*>(lds + base + off[v]);
+        *reinterpret_cast<typename Packed<Word, V>::P*>(a.data + cs) = tmp.p;
+      } else {
+        edge(cs);
+      }
+    }
+    return;
+  }
+  for (uint32_t t = tid; t < n16; t += nt) {
+    const int64_t cs = a0 + 16 * (int64_t)t;
+    if (V == 1 || cs < g0 || cs + 16 > g1) {
+      edge(cs);
+      continue;
+    }
+    // V words from word w0 of the run: the first one's block, cell and
+    // source from divisions, each next one's by one step of the digit
+    // odometer (or the next table entry)
+    const uint32_t w0 = (uint32_t)((cs - g0) / w);
+    uint32_t k = a.fnw.div(w0);
+    const uint32_t r = w0 - k * per;
+    uint32_t o = a.wpc == 1 ? r : a.fwpc.div(r);
+    uint32_t x = r - o * a.wpc;
+    DigitOdometer od;
+    if (a.ndig) od.init(a.frad, a.stride, a.ndig, o);
+    else od.src = blk_src(a, o);
+    Packed<Word, V> tmp;
+    if (a.wpc == 1 && a.ndig && od.dg[0] + V <= a.frad[0].d) {
+      // the V one-word cells differ only in their fastest digit
+      const uint32_t b = sh + k * (uint32_t)a.F + od.src * w;
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        tmp.e[v] = *reinterpret_cast<const Word*>(lds + lds_off(b + v * a.stride[0] * w, a.pad));
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        tmp.e[v] = *reinterpret_cast<const Word*>(
+            lds + lds_off(sh + k * (uint32_t)a.F + (od.src * a.wpc + x) * w, a.pad));
+        if (++x == a.wpc) {
+          x = 0;
+          if (++o == a.n) {  // the next block's first cell
+            o = 0;
+            ++k;
+            if (a.ndig) od.init(a.frad, a.stride, a.ndig, 0);
+            else od.src = blk_src(a, 0);
+          } else if (a.ndig) {
+            od.step(a.frad, a.stride, a.ndig);
+          } else {
+            od.src = blk_src(a, o);
+          }
+        }
+      }
+    }
+    *reinterpret_cast<typename Packed<Word, V>::P*>(a.data + cs) = tmp.p;
+  }
+}
+
 // f0: first block of the workgroup's K blocks
 template <typename Word, int V>
 __device__ __forceinline__ void block_body(const BlkArgs& a, const uint64_t f0, unsigned char* lds) {
   constexpr uint32_t w = sizeof(Word);
   const uint32_t nf = (uint32_t)((a.B - f0) < a.K ? (a.B - f0) : a.K);
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
+  if (a.dense) {
+    block_run<Word, V>(a, f0, nf, lds);
+    return;
+  }
+  // blocks of whole 16-byte chunks at an aligned address: the workgroup's nf
+  // blocks are one run of 16-byte chunks, each block padded on its own
   const char* g = a.data + a.base + f0 * a.F;
-  // the workgroup's nf blocks are contiguous: load them as one run of
-  // 16-byte chunks when its length allows (blocks of any size: a chunk may
-  // straddle two blocks when the LDS copy is dense)
   const uint32_t c16 = (uint32_t)(a.F / 16);
   if (a.vec16 && ((uint64_t)nf * a.F) % 16 == 0 && ((uintptr_t)g & 15) == 0) {
     const uint32_t n16 = (uint32_t)((uint64_t)nf * a.F / 16);
@@ -1971,12 +2330,8 @@ __device__ __forceinline__ void block_body(const BlkArgs& a, const uint64_t f0, 
       for (int u = 0; u < UNR; ++u) {
         const uint32_t t = t0 + u * nt;
         if (t < n16) {
-          if (a.dense) {
-            *reinterpret_cast<W16*>(lds + t * 16) = v[u];
-          } else {
-            const uint32_t k = a.fF16.div(t);
-            lds_put16(lds + k * a.Fl + lds_off((t - k * c16) * 16, a.pad), v[u]);
-          }
+          const uint32_t k = a.fF16.div(t);
+          lds_put16(lds + k * a.Fl + lds_off((t - k * c16) * 16, a.pad), v[u]);
         }
       }
     }
@@ -1985,13 +2340,35 @@ __device__ __forceinline__ void block_body(const BlkArgs& a, const uint64_t f0, 
     const uint32_t nw = nf * fw;
     for (uint32_t t = tid; t < nw; t += nt) {
       const uint32_t k = a.fFw.div(t);
-      *reinterpret_cast<Word*>(lds + k * a.Fl + lds_off((t - k * fw) * w, a.pad)) =
+      *reinterpret_cast<Word*>(lds + blk_lds(a, k, (t - k * fw) * w)) =
           reinterpret_cast<const Word*>(g)[t];
     }
   }
   __syncthreads();
   char* out = a.data + a.base + f0 * a.F;
-  if (V > 1 && ((uint64_t)nf * a.F) % 16 == 0 && ((uintptr_t)out & 15) == 0) {
+  if (V > 1 && a.P && nt % a.P == 0 && ((uint64_t)nf * a.F) % 16 == 0 && ((uintptr_t)out & 15) == 0) {
+    // one word per cell and a whole number of periods per P threads: thread
+    // tid always takes chunk tid % P of a period, so the LDS offsets of its
+    // V words, relative to the period, are computed once
+    const uint32_t total = (uint32_t)((uint64_t)nf * a.F / 16);
+    const uint32_t pos = tid % a.P, q = nt / a.P;
+    uint32_t off[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const uint32_t c = pos * V + (uint32_t)v;  // cell of the period
+      const uint32_t kb = a.fn.div(c);
+      const uint32_t src = blk_src(a, c - kb * a.n);
+      off[v] = kb * a.Fl + lds_off(src * w, a.pad);
+    }
+    uint32_t pi = tid / a.P;  // period of chunk t
+    for (uint32_t t = tid; t < total; t += nt, pi += q) {
+      const uint32_t base = pi * a.Fl;
+      Packed<Word, V> tmp;
+#pragma unroll
+      for (int v = 0; v < V; ++v) tmp.e[v] = *reinterpret_cast<const Word*>(lds + base + off[v]);
+      *reinterpret_cast<typename Packed<Word, V>::P*>(out + (uint64_t)t * 16) = tmp.p;
+    }
+  } else if (V > 1 && ((uint64_t)nf * a.F) % 16 == 0 && ((uintptr_t)out & 15) == 0) {
     // 16-byte stores: chunk t of the run is words t*V .. t*V+V-1 (cells of
     // one or several words, possibly of two blocks).  The first word's
     // block, cell and source come from divisions; each following word's from
@@ -2014,15 +2391,13 @@ __device__ __forceinline__ void block_body(const BlkArgs& a, const uint64_t f0, 
         // the V one-word cells differ only in their fastest digit
 #pragma unroll
         for (int v = 0; v < V; ++v)
-          tmp.e[v] = *reinterpret_cast<const Word*>(
-              lds + k * a.Fl + lds_off((od.src + v * a.stride[0]) * w, a.pad));
+          tmp.e[v] = *reinterpret_cast<const Word*>(lds + blk_lds(a, k, (od.src + v * a.stride[0]) * w));
         *reinterpret_cast<typename Packed<Word, V>::P*>(out + (uint64_t)t * 16) = tmp.p;
         continue;
       }
 #pragma unroll
       for (int v = 0; v < V; ++v) {
-        tmp.e[v] = *reinterpret_cast<const Word*>(lds + k * a.Fl +
-                                                lds_off((od.src * a.wpc + x) * w, a.pad));
+        tmp.e[v] = *reinterpret_cast<const Word*>(lds + blk_lds(a, k, (od.src * a.wpc + x) * w));
         if (++x == a.wpc) {
           x = 0;
           if (++o == a.n) {  // the next block's first cell
@@ -2048,8 +2423,8 @@ __device__ __forceinline__ void block_body(const BlkArgs& a, const uint64_t f0, 
       const uint32_t o = a.wpc == 1 ? r : a.fwpc.div(r);
       const uint32_t x = r - o * a.wpc;
       const uint32_t src = blk_src(a, o);
-      reinterpret_cast<Word*>(out)[(uint64_t)k * per + r] = *reinterpret_cast<const Word*>(
-          lds + k * a.Fl + lds_off((src * a.wpc + x) * w, a.pad));
+      reinterpret_cast<Word*>(out)[(uint64_t)k * per + r] =
+          *reinterpret_cast<const Word*>(lds + blk_lds(a, k, (src * a.wpc + x) * w));
     }
   }
 }
@@ -2090,6 +2465,8 @@ hipError_t launch_block_w(BlkArgs a, int vec, int64_t lds, int64_t groups, const
     const int64_t words = (lds + 15) / 16;
     threads = (int)std::min<int64_t>(512, std::max<int64_t>(64, ((words + UNR - 1) / UNR + 63) / 64 * 64));
   }
+  // pattern stores want a whole number of periods per workgroup's threads
+  if (vec && V > 1 && a.P) threads = std::max<int>((int)a.P, threads / (int)a.P * (int)a.P);
   if (vec && V > 1)
     hipLaunchKernelGGL((k_block<Word, V>), dim3((uint32_t)groups), dim3(threads), (size_t)lds, stream,
                        a, cfg.xcd_remap);
@@ -2124,23 +2501,39 @@ hipError_t prepare_block_args(const Op& op, char* data, const LaunchCfg& cfg, Bl
   a.vec16 = base % 16 == 0 ? 1 : 0;
   a.idx16 = t.cells <= 65536 ? 1 : 0;
   const int64_t V = w < 16 ? 16 / w : 1;
-  const int vec = V > 1 && a.vec16;
+  const int vec = V > 1;
   a.chunks = (uint32_t)op.R;
-  // blocks whose size is not a multiple of 16 bytes lie in LDS back to back
-  // (16-byte loads may then straddle two blocks); otherwise padding keeps
-  // cells of up to 16 bytes whole (pad intervals are >= 64 bytes) and LDS
-  // words aligned (only 1-, 2- and 4-byte words are padded)
-  a.dense = F % 16 ? 1 : 0;
-  a.pad = (!a.dense && t.pad_log2 && 64 % op.U == 0 && w <= 4) ? (uint32_t)t.pad_log2 : 0;
+  // blocks whose size is not a multiple of 16 bytes, or that do not start
+  // 16-byte aligned, lie in LDS back to back as one run kept at its address
+  // modulo 16 (dense), padded as one byte range; otherwise each block is
+  // padded on its own.  Padding keeps cells of up to 16 bytes whole (pad
+  // intervals are >= 64 bytes and cells divide 64 bytes) and LDS words
+  // aligned (only 1-, 2- and 4-byte words are padded)
+  a.dense = (F % 16 || base % 16) ? 1 : 0;
+  a.pad = (t.pad_log2 && 64 % op.U == 0 && w <= 4) ? (uint32_t)t.pad_log2 : 0;
   const int64_t Fl = a.pad ? F + (F >> a.pad) * 4 + 4 : F;
   a.Fl = (uint32_t)(a.dense ? F : (Fl + 15) / 16 * 16);
-  if (a.Fl > 160 * 1024) return hipErrorInvalidValue;
-  int64_t K = pick_K(op.B, a.Fl, cfg.lds_target);
-  while (K > 1 && K * a.Fl > 160 * 1024) --K;
+  // LDS bytes of K blocks (dense: the run, the partial chunks at its two
+  // ends and the pads)
+  auto lds_of = [&](int64_t K) -> int64_t {
+    if (!a.dense) return K * (int64_t)a.Fl;
+    const int64_t run = K * F + 32;
+    return (run + (a.pad ? (run >> a.pad) * 4 + 4 : 0) + 15) / 16 * 16;
+  };
+  if (lds_of(1) > 160 * 1024) return hipErrorInvalidValue;
+  {
+    const int64_t Pc = F / gcd64(F, 16);  // 16-byte chunks per period of lcm(F, 16) bytes
+    a.P = (a.wpc == 1 && V > 1 && Pc <= 512 &&
+           (!a.dense || !a.pad || (Pc * 16) % (1ll << a.pad) == 0))
+              ? (uint32_t)Pc
+              : 0;
+  }
+  int64_t K = pick_K(op.B, a.dense ? Fl : (int64_t)a.Fl, cfg.lds_target);
+  while (K > 1 && lds_of(K) > 160 * 1024) --K;
   // a multiple of the blocks that make 16-byte runs and whole 16-byte stores
   // (past the LDS target when that many blocks fit the workgroup's LDS)
-  int64_t mK = 16 / gcd64(F, 16);
-  if (K < mK && mK <= op.B && mK * (int64_t)a.Fl <= 160 * 1024) K = mK;
+  int64_t mK = a.dense ? 1 : 16 / gcd64(F, 16);
+  if (K < mK && mK <= op.B && lds_of(mK) <= 160 * 1024) K = mK;
   if (K >= mK) K -= K % mK;
   a.K = (uint32_t)K;
   a.fnw.init(a.n * a.wpc);
@@ -2158,7 +2551,7 @@ hipError_t prepare_block_args(const Op& op, char* data, const LaunchCfg& cfg, Bl
   if (groups >= (1ll << 31)) return hipErrorInvalidValue;
   *w_out = w;
   *vec_out = vec;
-  *lds_out = K * (int64_t)a.Fl;
+  *lds_out = lds_of(K);
   *groups_out = groups;
   return hipSuccess;
 }

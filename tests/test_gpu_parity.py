@@ -320,3 +320,43 @@ def test_shifted_cycle_passes(torch_cuda):
         E.set_tuning("cycle_lmax", 0)
         E.set_tuning("cycle_variant", 0)
         E.set_tuning("regroup", regroup)
+
+
+DENSE_BLOCK_CASES = [([71, 30, 277, 4], [3, 4, 2, 1], 1),     # blocks of 19,667 and 33,240 bytes
+                     ([67, 8, 54, 45, 2, 8], [3, 2, 6, 4, 5, 1], 1),  # 28,944-byte blocks
+                     ([13, 4, 3, 63, 505], [3, 4, 2, 5, 1], 2),     # 9,828 two-byte cells
+                     ([57, 137, 340], [2, 1, 3], 4),                # 7,809 four-byte cells
+                     ([63, 2, 8661, 3], [2, 1, 3, 4], 1),           # 126-byte blocks (pattern)
+                     ([4, 2, 38591, 9], [2, 1, 4, 3], 1),           # 2,664-byte blocks
+                     ([58, 2, 12336], [3, 2, 1], 1),                # 29,812-byte blocks
+                     ([6, 17, 2, 21875], [2, 3, 4, 1], 2)]          # 840-byte blocks (pattern)
+
+
+def test_dense_block_passes(torch_cuda):
+    # fused blocks whose size is not a multiple of 16 bytes, or at a data
+    # pointer one element past an aligned one: the workgroup's run of blocks
+    # sits in LDS at its address modulo 16, interior 16-byte chunks are
+    # gathered whole (digit odometer, gather table, or per-thread pattern
+    # offsets for small blocks) and the run's edge chunks word by word
+    import eithot_amd as E
+    from oracle import pyoracle as O
+    try:
+        for digits in (1, 0):
+            E.set_tuning("block_digits", digits)
+            for dims, perm, elem in DENSE_BLOCK_CASES:
+                ops, _ = E.plan_describe(dims, perm, elem, 1, 0.05)
+                assert any(o["kind"] == 5 for o in ops), (dims, perm, ops)
+                got, want, _ = run_case(torch_cuda, dims, perm, elem, seed=20, alpha=0.05)
+                assert_bitexact(got, want, f"digits {digits} {dims} {perm} elem {elem}")
+                vol = int(np.prod(dims))
+                host = O.fill(vol + 1, elem, 21, 1)
+                dev = torch_cuda.from_numpy(host.copy()).to("cuda:0")
+                print(f"case offset 1 digits={digits} dims={dims} perm={perm} elem={elem}", flush=True)
+                E.transpose(dev, 1, len(dims), dims, [p - 1 for p in perm], elem, 1, 0.05)
+                torch_cuda.cuda.synchronize()
+                want = host.copy()
+                want[elem:] = O.permute(np.ascontiguousarray(host[elem:]), dims, [p - 1 for p in perm],
+                                        elem)
+                assert_bitexact(dev.cpu().numpy(), want, f"offset digits {digits} {dims} {perm}")
+    finally:
+        E.set_tuning("block_digits", 1)
