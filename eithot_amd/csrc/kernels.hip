@@ -480,12 +480,15 @@ __device__ __forceinline__ void col_wide(const Args<Idx>& a, const Idx f0, const
     const Idx x = (Idx)(b0 - g);
     Idx jj = a.fU.div(x);
     Idx xu = x - jj * (Idx)a.us;
-    Idx s = col_src<Idx, MAP>(a, i, f.row + jj);
-    // LDS byte of (row r, byte y of the row)
-    auto at = [&](Idx r, Idx y) -> const Word* {
-      const Idx sh = (Idx)((d0 + f.addr + (uint64_t)r * a.CU) & 15);
-      return reinterpret_cast<const Word*>(lds + k * a.fpl + r * a.rs + sh + y);
+    // LDS start of source row r's segment: k*fpl + r*rs plus the segment's
+    // address modulo 16, (f15 + r*cu15) mod 16 in 32 bits.  Byte y of the
+    // output row (column jj, byte xu) is byte y of the source row's segment,
+    // so only the row base changes, once per column.
+    const uint32_t f15 = (uint32_t)((d0 + f.addr) & 15), cu15 = (uint32_t)(a.CU & 15);
+    auto rowb = [&](Idx r) -> const unsigned char* {
+      return lds + k * a.fpl + r * a.rs + ((f15 + (uint32_t)r * cu15) & 15);
     };
+    const unsigned char* rb = rowb(col_src<Idx, MAP>(a, i, f.row + jj));
     if (b0 == cs && b1 == cs + 16) {
       Packed<Word, V> tmp;
 #pragma unroll
@@ -493,20 +496,21 @@ __device__ __forceinline__ void col_wide(const Args<Idx>& a, const Idx f0, const
         if (xu == (Idx)a.us) {
           xu = 0;
           ++jj;
-          s = col_src<Idx, MAP>(a, i, f.row + jj);
+          rb = rowb(col_src<Idx, MAP>(a, i, f.row + jj));
         }
-        tmp.e[q] = *at(s, jj * (Idx)a.us + xu);
+        tmp.e[q] = *reinterpret_cast<const Word*>(rb + x + (Idx)q * w);
         xu += w;
       }
       *reinterpret_cast<typename Packed<Word, V>::P*>(a.data + cs) = tmp.p;
     } else {
-      for (int64_t b = b0; b < b1; b += w) {
+      Idx y = x;
+      for (int64_t b = b0; b < b1; b += w, y += w) {
         if (xu == (Idx)a.us) {
           xu = 0;
           ++jj;
-          s = col_src<Idx, MAP>(a, i, f.row + jj);
+          rb = rowb(col_src<Idx, MAP>(a, i, f.row + jj));
         }
-        *reinterpret_cast<Word*>(a.data + b) = *at(s, jj * (Idx)a.us + xu);
+        *reinterpret_cast<Word*>(a.data + b) = *reinterpret_cast<const Word*>(rb + y);
         xu += w;
       }
     }
@@ -1849,8 +1853,9 @@ hipError_t prepare_cycle_args(const Op& op, char* data, char* scratch, int64_t s
 // ---------------------------------------------------------------------------
 // Cycle following of cells that are not whole aligned 16-byte words
 // (k_cycle_u).  An item owns the cell-relative bytes [r0, r0 + n) of every
-// cell of its segment, r0 = g*G with G = (lanes - 2) * 16, so items that
-// share cells never touch each other's bytes.  At each position the item's
+// cell of its segment, r0 = g*G with G at most (lanes - 2) * 16 bytes and a
+// whole number of words, so items that share cells never touch each other's
+// bytes and every range starts and ends on a word.  At each position the item's
 // lanes cover the aligned 16-byte chunks of the destination's range (at
 // most lanes - 1 of them): lane l takes chunk X = (first chunk) + 16*l.
 // Its bytes come from the source range at the same cell-relative offsets:
@@ -1860,7 +1865,9 @@ hipError_t prepare_cycle_args(const Op& op, char* data, char* scratch, int64_t s
 // above it with a wave shuffle (ds_bpermute), so each source byte is
 // requested once.  Bytes read outside the range are never stored.  A
 // destination chunk wholly inside the range is stored with one 16-byte
-// store, the range's end chunks as aligned pieces of 8, 4, 2 and 1 bytes.
+// store, the range's end chunks as at most six aligned pieces of 8, 4, 2 and
+// 1 bytes, none narrower than a word.  Offsets are 32-bit when the pass
+// spans less than 2 GB.
 // Source chunks that would reach outside the pass's bytes (or the boundary
 // area) are gathered byte by byte.  The step order and hazards are those
 // of k_cycle: within a wave, every load of a batch completes before any of
@@ -1918,38 +1925,42 @@ __device__ __forceinline__ W16 shfl_next(const W16& v, int width) {
   return o;
 }
 
-// Stores the bytes of chunk v (at offset X from data) that lie in
-// [x0, x1), x0 < x1 within the chunk: 16 bytes at once when that is the
-// whole chunk, else as aligned pieces of 8, 4, 2 and 1 bytes.
-__device__ __forceinline__ void cycu_store(char* data, int64_t X, int64_t x0, int64_t x1,
-                                           const W16& v) {
+// Stores the bytes of chunk v (at offset X from data, 16-byte aligned) that
+// lie in [x0, x1), x0 < x1 within the chunk and both multiples of the word:
+// 16 bytes at once when that is the whole chunk, else as aligned pieces of
+// 8, 4, 2 and 1 bytes (at most six; none narrower than the word).
+template <typename Word, typename Off>
+__device__ __forceinline__ void cycu_store(char* data, Off X, Off x0, Off x1, const W16& v) {
+  constexpr uint32_t w = sizeof(Word);
   if (x0 == X && x1 == X + 16) {
     *reinterpret_cast<W16*>(data + X) = v;
     return;
   }
-  const uint64_t lo = (uint64_t)v.x | ((uint64_t)v.y << 32), hi = (uint64_t)v.z | ((uint64_t)v.w << 32);
-  const uintptr_t d0 = (uintptr_t)data;
-  for (int64_t x = x0; x < x1;) {
-    const uint32_t o = (uint32_t)(x - X);  // byte of the chunk
-    const uint64_t bits = (o < 8 ? lo >> (8 * o) : hi >> (8 * (o - 8)));
-    const uintptr_t at = d0 + (uint64_t)x;
-    if ((at & 7) == 0 && x + 8 <= x1) {
-      *reinterpret_cast<uint64_t*>(data + x) = bits;
-      x += 8;
-    } else if ((at & 3) == 0 && x + 4 <= x1) {
-      *reinterpret_cast<uint32_t*>(data + x) = (uint32_t)bits;
-      x += 4;
-    } else if ((at & 1) == 0 && x + 2 <= x1) {
-      *reinterpret_cast<uint16_t*>(data + x) = (uint16_t)bits;
-      x += 2;
-    } else {
-      data[x] = (char)bits;
-      x += 1;
-    }
-  }
+  uint32_t o = (uint32_t)(x0 - X);
+  const uint32_t e = (uint32_t)(x1 - X);
+  char* c = data + X;
+  auto dw = [&](uint32_t k) -> uint32_t { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; };
+  auto st1 = [&](uint32_t q) { c[q] = (char)(dw(q >> 2) >> (8 * (q & 3))); };
+  auto st2 = [&](uint32_t q) { *reinterpret_cast<uint16_t*>(c + q) = (uint16_t)(dw(q >> 2) >> (8 * (q & 3))); };
+  auto st4 = [&](uint32_t q) { *reinterpret_cast<uint32_t*>(c + q) = dw(q >> 2); };
+  auto st8 = [&](uint32_t q) {
+    *reinterpret_cast<uint64_t*>(c + q) =
+        q ? ((uint64_t)v.w << 32 | v.z) : ((uint64_t)v.y << 32 | v.x);
+  };
+  if (w <= 1 && (o & 1) && o + 1 <= e) { st1(o); o += 1; }
+  if (w <= 2 && (o & 2) && o + 2 <= e) { st2(o); o += 2; }
+  if (w <= 4 && (o & 4) && o + 4 <= e) { st4(o); o += 4; }
+  if ((o & 8) && o + 8 <= e) { st8(o); o += 8; }
+  if (o + 8 <= e) { st8(o); o += 8; }
+  if (w <= 4 && o + 4 <= e) { st4(o); o += 4; }
+  if (w <= 2 && o + 2 <= e) { st2(o); o += 2; }
+  if (w <= 1 && o + 1 <= e) { st1(o); }
 }
 
-template <int D>
+// Word: the widest word dividing the cell and the tensor's address (below
+// 16 bytes); Off: int32_t when every offset of the pass (and of the
+// boundary area) stays below 2^31, else int64_t.
+template <typename Word, typename Off, int D>
 __global__ __launch_bounds__(256) void k_cycle_u(CycArgs a, CycUArgs u) {
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int L = 1 << u.lanes_log2;
@@ -1962,73 +1973,73 @@ __global__ __launch_bounds__(256) void k_cycle_u(CycArgs a, CycUArgs u) {
   const uint64_t g = item - rest * u.groups;
   const uint64_t b = a.fnseg.div(rest);
   const uint64_t sidx = rest - b * a.nseg;
-  const int64_t r0 = (int64_t)(g * u.G);
-  const int64_t n = (int64_t)a.U - r0 < (int64_t)u.G ? (int64_t)a.U - r0 : (int64_t)u.G;
-  const int64_t m = (int64_t)(a.base + b * a.RCU);  // the matrix, offset from data
-  const int64_t lo = (int64_t)a.base, hi = (int64_t)(a.base + a.B * a.RCU);
-  const int64_t blo = 0, bhi = (int64_t)(a.B * a.nslots * a.U);
-  const int64_t U = (int64_t)a.U;
-  const uintptr_t d0 = (uintptr_t)a.data;
-  // lane's destination chunk of the range that starts at offset r (from data)
-  auto chunk = [&](int64_t r) { return r - (int64_t)((d0 + (uint64_t)r) & 15) + 16 * (int64_t)lane; };
+  const Off U = (Off)a.U;
+  const Off r0 = (Off)(g * u.G);
+  const Off n = U - r0 < (Off)u.G ? U - r0 : (Off)u.G;
+  const Off m = (Off)(a.base + b * a.RCU) + r0;  // byte r0 of cell 0 of the matrix
+  const Off lo = (Off)a.base, hi = (Off)(a.base + a.B * a.RCU);
+  const Off bhi = (Off)(a.B * a.nslots * a.U);
+  const uint32_t dmod = (uint32_t)((uintptr_t)a.data & 15), bmod = (uint32_t)((uintptr_t)a.boundary & 15);
+  const Off l16 = (Off)(16 * lane);
+  // lane's destination chunk of the range that starts at offset dr (from data)
+  auto chunk = [&](Off dr) -> Off { return dr - (Off)((dmod + (uint32_t)dr) & 15) + l16; };
   // the source chunk this lane loads for destination range dr taking the
-  // range at src of sbase (valid bytes [slo, shi)); *xs: the source offset
-  // of the lane's destination chunk, *sig: its misalignment.  A lane's chunk
-  // feeds its own destination chunk and the one of the lane below it, so
-  // only lanes whose destination chunk starts before dr + n + 16 load.
-  auto load = [&](int64_t dr, const char* sbase, int64_t src, int64_t slo, int64_t shi, int64_t* xs,
+  // range at sr of sb (address mod 16: smod; valid bytes [slo, shi)); *s0:
+  // its offset, *sig: the misalignment of the lane's destination chunk in
+  // it.  A lane's chunk feeds its own destination chunk and the one of the
+  // lane below it, so only lanes whose chunk starts before dr + n + 16 load.
+  auto load = [&](Off dr, const char* sb, uint32_t smod, Off sr, Off slo, Off shi, Off* s0,
                   uint32_t* sig) -> W16 {
-    const int64_t X = chunk(dr);
-    *xs = X - dr + src;
-    *sig = (uint32_t)(((uintptr_t)sbase + (uint64_t)*xs) & 15);
-    const int64_t s0 = *xs - *sig;
+    const Off X = chunk(dr);
+    const Off xs = X - dr + sr;
+    *sig = (smod + (uint32_t)xs) & 15;
+    *s0 = xs - (Off)*sig;
     W16 v = {0, 0, 0, 0};
-    if (X < dr + n + 16 && s0 >= slo && s0 + 16 <= shi) v = *reinterpret_cast<const W16*>(sbase + s0);
+    if (X < dr + n + 16 && *s0 >= slo && *s0 + 16 <= shi) v = *reinterpret_cast<const W16*>(sb + *s0);
     return v;
   };
   // the 16 bytes of the lane's destination chunk from its loaded chunk c
   // (all lanes of the item call this together: it shuffles); a lane whose
   // destination chunk starts at or past dr + n stores nothing and gathers
   // nothing
-  auto assemble = [&](const W16& c, int64_t dr, int64_t xs, uint32_t sig, const char* sbase,
-                      int64_t slo, int64_t shi) -> W16 {
+  auto assemble = [&](const W16& c, Off dr, Off s0, uint32_t sig, const char* sb, Off slo,
+                      Off shi) -> W16 {
     const W16 nx = shfl_next(c, L);
-    const int64_t s0 = xs - sig;
     if (s0 >= slo && s0 + 32 <= shi) return sig ? funnel16(c, nx, sig) : c;
     if (chunk(dr) >= dr + n) return c;
-    return gather16(sbase, xs, slo, shi);
+    return gather16(sb, (int64_t)(s0 + (Off)sig), (int64_t)slo, (int64_t)shi);
   };
   const CycleSeg sg = a.segs[sidx];
   uint32_t p = sg.start;
   // a whole cycle's last write takes the old start cell: read it first
   W16 fin = {0, 0, 0, 0};
   if (sg.src_slot < 0) {
-    const int64_t dl = m + (int64_t)sg.last * U + r0;
-    int64_t xs;
+    const Off dl = m + (Off)sg.last * U;
+    Off s0;
     uint32_t sig;
-    const W16 c = load(dl, a.data, m + (int64_t)p * U + r0, lo, hi, &xs, &sig);
-    fin = assemble(c, dl, xs, sig, a.data, lo, hi);
+    const W16 c = load(dl, a.data, dmod, m + (Off)p * U, lo, hi, &s0, &sig);
+    fin = assemble(c, dl, s0, sig, a.data, lo, hi);
   }
   const char* bnd = a.boundary;
+  const Off bsrc = sg.src_slot < 0 ? 0 : (Off)((b * a.nslots + (uint64_t)sg.src_slot) * a.U) + r0;
   for (uint32_t t = 0; t < sg.len; t += D) {
     uint32_t q[D];
     uint32_t pp = p;
 #pragma unroll
     for (int k = 0; k < D; ++k) q[k] = pp = cyc_src(a, pp);
     W16 v[D];
-    int64_t xs[D];
+    Off s0[D];
     uint32_t sig[D];
     // the batch's loads (the last write of a cut segment reads its saved cell)
     uint32_t dst = p;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       if (t + k < sg.len && !(t + k + 1 == sg.len && sg.src_slot < 0)) {
-        const int64_t dr = m + (int64_t)dst * U + r0;
+        const Off dr = m + (Off)dst * U;
         if (t + k + 1 < sg.len)
-          v[k] = load(dr, a.data, m + (int64_t)q[k] * U + r0, lo, hi, &xs[k], &sig[k]);
+          v[k] = load(dr, a.data, dmod, m + (Off)q[k] * U, lo, hi, &s0[k], &sig[k]);
         else
-          v[k] = load(dr, bnd, (int64_t)((b * a.nslots + (uint64_t)sg.src_slot) * a.U) + r0, blo,
-                      bhi, &xs[k], &sig[k]);
+          v[k] = load(dr, bnd, bmod, bsrc, 0, bhi, &s0[k], &sig[k]);
       }
       dst = q[k];
     }
@@ -2036,13 +2047,13 @@ __global__ __launch_bounds__(256) void k_cycle_u(CycArgs a, CycUArgs u) {
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       if (t + k < sg.len) {
-        const int64_t dr = m + (int64_t)dst * U + r0;
+        const Off dr = m + (Off)dst * U;
         if (t + k + 1 < sg.len)
-          v[k] = assemble(v[k], dr, xs[k], sig[k], a.data, lo, hi);
+          v[k] = assemble(v[k], dr, s0[k], sig[k], a.data, lo, hi);
         else if (sg.src_slot < 0)
           v[k] = fin;
         else
-          v[k] = assemble(v[k], dr, xs[k], sig[k], bnd, blo, bhi);
+          v[k] = assemble(v[k], dr, s0[k], sig[k], bnd, 0, bhi);
       }
       dst = q[k];
     }
@@ -2050,10 +2061,10 @@ __global__ __launch_bounds__(256) void k_cycle_u(CycArgs a, CycUArgs u) {
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       if (t + k < sg.len) {
-        const int64_t dr = m + (int64_t)dst * U + r0;
-        const int64_t X = chunk(dr);
-        const int64_t x0 = X < dr ? dr : X, x1 = X + 16 < dr + n ? X + 16 : dr + n;
-        if (x0 < x1) cycu_store(a.data, X, x0, x1, v[k]);
+        const Off dr = m + (Off)dst * U;
+        const Off X = chunk(dr);
+        const Off x0 = X < dr ? dr : X, x1 = X + 16 < dr + n ? X + 16 : dr + n;
+        if (x0 < x1) cycu_store<Word, Off>(a.data, X, x0, x1, v[k]);
       }
       dst = q[k];
     }
@@ -2069,21 +2080,28 @@ hipError_t launch_cycle_u(CycArgs a, hipStream_t stream) {
     hipLaunchKernelGGL((k_cycle_save<Word>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
   }
   CycUArgs u;
-  // lanes per item: G = (lanes - 2) * 16 cell bytes need at most lanes - 1
+  // lanes per item: G <= (lanes - 2) * 16 cell bytes need at most lanes - 1
   // destination chunks, so the last lane only hands its chunk down
   uint32_t ll = 2;
   while (ll < 6 && ((uint64_t)((1u << ll) - 2) * 16) < a.U) ++ll;
   u.lanes_log2 = ll;
   const uint64_t gmax = (uint64_t)((1u << ll) - 2) * 16;
-  // cells larger than that are cut into equal groups (the last one may be
-  // shorter by less than a group count of bytes)
+  // cells larger than that are cut into equal groups of whole words (the
+  // last one may be shorter); gmax is a multiple of 16, so G stays within it
+  constexpr uint64_t w = sizeof(Word);
   u.groups = (a.U + gmax - 1) / gmax;
-  u.G = (a.U + u.groups - 1) / u.groups;
+  u.G = ((a.U + u.groups - 1) / u.groups + w - 1) / w * w;
   u.fgroups.init(u.groups);
   a.items = a.B * a.nseg * u.groups;
   const uint64_t blocks = ((a.items << ll) + 255) / 256;
   if (blocks >= (1ull << 31)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_cycle_u<8>), dim3((uint32_t)blocks), dim3(256), 0, stream, a, u);
+  // every offset the kernel forms lies within [-16, end of the pass +
+  // 64 lanes' chunks + 32)
+  const uint64_t span = std::max<uint64_t>(a.base + a.B * a.RCU, a.B * a.nslots * a.U) + 2048;
+  if (span < (1ull << 31))
+    hipLaunchKernelGGL((k_cycle_u<Word, int32_t, 8>), dim3((uint32_t)blocks), dim3(256), 0, stream, a, u);
+  else
+    hipLaunchKernelGGL((k_cycle_u<Word, int64_t, 8>), dim3((uint32_t)blocks), dim3(256), 0, stream, a, u);
   return hipGetLastError();
 }
 
@@ -2273,12 +2291,31 @@ __device__ __forceinline__ void block_run(const BlkArgs& a, const uint64_t f0, c
     if (a.ndig) od.init(a.frad, a.stride, a.ndig, o);
     else od.src = blk_src(a, o);
     Packed<Word, V> tmp;
-    if (a.wpc == 1 && a.ndig && od.dg[0] + V <= a.frad[0].d) {
-      // the V one-word cells differ only in their fastest digit
-      const uint32_t b = sh + k * (uint32_t)a.F + od.src * w;
+    if (a.wpc == 1 && a.ndig && a.frad[0].d >= (uint32_t)V) {
+      // one-word cells whose fastest output digit spans at least V cells:
+      // the chunk's V words lie in at most two runs of that digit, each an
+      // arithmetic sequence in LDS; the second run starts one odometer step
+      // past the first one's last cell (the next block's first cell when the
+      // block ends there)
+      const uint32_t S = a.stride[0] * w;
+      const uint32_t k1 = a.frad[0].d - od.dg[0];  // cells left in the first run
+      const uint32_t base0 = sh + k * (uint32_t)a.F + od.src * w;
+      uint32_t base1 = base0;
+      if (k1 < (uint32_t)V) {
+        if (o + k1 == a.n) {
+          base1 = sh + (k + 1) * (uint32_t)a.F;
+        } else {
+          od.src += (k1 - 1) * a.stride[0];
+          od.dg[0] = a.frad[0].d - 1;
+          od.step(a.frad, a.stride, a.ndig);
+          base1 = sh + k * (uint32_t)a.F + od.src * w;
+        }
+      }
+      const uint32_t b1 = base1 - k1 * S;  // word v of the second run: b1 + v*S
 #pragma unroll
       for (int v = 0; v < V; ++v)
-        tmp.e[v] = *reinterpret_cast<const Word*>(lds + lds_off(b + v * a.stride[0] * w, a.pad));
+        tmp.e[v] = *reinterpret_cast<const Word*>(
+            lds + lds_off(((uint32_t)v < k1 ? base0 : b1) + (uint32_t)v * S, a.pad));
     } else {
 #pragma unroll
       for (int v = 0; v < V; ++v) {
@@ -2387,11 +2424,31 @@ __device__ __forceinline__ void block_body(const BlkArgs& a, const uint64_t f0, 
       if (a.ndig) od.init(a.frad, a.stride, a.ndig, o);
       else od.src = blk_src(a, o);
       Packed<Word, V> tmp;
-      if (a.wpc == 1 && a.ndig && od.dg[0] + V <= a.frad[0].d) {
-        // the V one-word cells differ only in their fastest digit
+      if (a.wpc == 1 && a.ndig && a.frad[0].d >= (uint32_t)V) {
+        // one-word cells whose fastest output digit spans at least V cells:
+        // at most two runs of that digit (see block_run)
+        const uint32_t S = a.stride[0] * w;
+        const uint32_t k1 = a.frad[0].d - od.dg[0];
+        const uint32_t y0 = od.src * w, blk0 = k * a.Fl;
+        uint32_t y1 = y0, blk1 = blk0;
+        if (k1 < (uint32_t)V) {
+          if (o + k1 == a.n) {
+            y1 = 0;
+            blk1 = (k + 1) * a.Fl;
+          } else {
+            od.src += (k1 - 1) * a.stride[0];
+            od.dg[0] = a.frad[0].d - 1;
+            od.step(a.frad, a.stride, a.ndig);
+            y1 = od.src * w;
+          }
+        }
+        const uint32_t z1 = y1 - k1 * S;
 #pragma unroll
-        for (int v = 0; v < V; ++v)
-          tmp.e[v] = *reinterpret_cast<const Word*>(lds + blk_lds(a, k, (od.src + v * a.stride[0]) * w));
+        for (int v = 0; v < V; ++v) {
+          const bool first = (uint32_t)v < k1;
+          tmp.e[v] = *reinterpret_cast<const Word*>(
+              lds + (first ? blk0 : blk1) + lds_off((first ? y0 : z1) + (uint32_t)v * S, a.pad));
+        }
         *reinterpret_cast<typename Packed<Word, V>::P*>(out + (uint64_t)t * 16) = tmp.p;
         continue;
       }
