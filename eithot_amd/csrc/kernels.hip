@@ -256,12 +256,81 @@ __device__ __forceinline__ void tile_wide(const Args<Idx>& a, const Idx f0, cons
   }
 }
 
+// Wide tile pass with one LDS copy (wide == 3: matrices too large for two
+// copies): the run is loaded as in tile_wide, then every aligned 16-byte
+// output chunk gathers its words straight from that copy.  Output byte
+// (matrix k, output cell j*R + i, byte xu) is input byte k*RCU + i*CU +
+// j*U + xu; consecutive words step a four-digit odometer (xu, i, j, k)
+// whose carries are selects, not branches.
+template <typename Word, typename Idx>
+__device__ __forceinline__ void tile_gather(const Args<Idx>& a, const Idx f0, const Idx nf,
+                                            unsigned char* lds) {
+  constexpr uint32_t w = sizeof(Word);
+  constexpr int V = 16 / (int)sizeof(Word);
+  const Idx tid = threadIdx.x, nt = blockDim.x;
+  const uintptr_t d0 = (uintptr_t)a.data;
+  const int64_t g0 = (int64_t)(a.base + (uint64_t)f0 * a.RCU);
+  const int64_t g1 = g0 + (int64_t)nf * (int64_t)a.RCU;
+  const int64_t a0 = g0 - (int64_t)((d0 + (uint64_t)g0) & 15);
+  const uint32_t sh = (uint32_t)(g0 - a0);
+  run_to_lds<Word, Idx>(a, g0, g1, a0, lds);
+  __syncthreads();
+  const uint32_t U = (uint32_t)a.U, CU = (uint32_t)a.CU, RCU = (uint32_t)a.RCU;
+  const uint32_t R = (uint32_t)a.R, C = (uint32_t)a.C;
+  const Idx n = (Idx)((g1 - a0 + 15) / 16);
+  for (Idx t = tid; t < n; t += nt) {
+    const int64_t cs = a0 + 16 * (int64_t)t;
+    const int64_t b0 = cs < g0 ? g0 : cs, b1 = cs + 16 < g1 ? cs + 16 : g1;
+    // the first word's matrix, output cell and byte
+    const Idx x = (Idx)(b0 - g0);
+    const Idx c = a.fU.div(x);
+    const Idx k = a.fld.div(c);  // ld_per == R*C
+    const Idx r = c - k * (Idx)(R * C);
+    const Idx jq = a.fR.div(r);
+    uint32_t i = (uint32_t)(r - jq * R), j = (uint32_t)jq, xu = (uint32_t)(x - c * U);
+    uint32_t src = sh + (uint32_t)k * RCU + i * CU + j * U + xu;
+    auto step = [&]() {
+      xu += w;
+      src += w;
+      const bool e1 = xu == U;
+      xu = e1 ? 0 : xu;
+      i += e1 ? 1 : 0;
+      src += e1 ? CU - U : 0;
+      const bool e2 = i == R;
+      i = e2 ? 0 : i;
+      j += e2 ? 1 : 0;
+      src += e2 ? U - R * CU : 0;
+      const bool e3 = j == C;
+      j = e3 ? 0 : j;
+      src += e3 ? RCU - C * U : 0;
+    };
+    if (b0 == cs && b1 == cs + 16) {
+      Packed<Word, V> tmp;
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        tmp.e[v] = *reinterpret_cast<const Word*>(lds + src);
+        step();
+      }
+      *reinterpret_cast<typename Packed<Word, V>::P*>(a.data + cs) = tmp.p;
+    } else {
+      for (int64_t b = b0; b < b1; b += w) {
+        *reinterpret_cast<Word*>(a.data + b) = *reinterpret_cast<const Word*>(lds + src);
+        step();
+      }
+    }
+  }
+}
+
 template <typename Word, typename Idx, int VEC>
 __device__ __forceinline__ void tile_body(const Args<Idx>& a, const Idx f0, unsigned char* lds) {
   const Idx nf = (a.nfp - f0) < a.K ? (a.nfp - f0) : a.K;
   const Idx tid = threadIdx.x, nt = blockDim.x;
   constexpr Idx w = sizeof(Word);
   if constexpr (sizeof(Word) < 16) {
+    if (a.wide == 3) {
+      tile_gather<Word, Idx>(a, f0, nf, lds);
+      return;
+    }
     if (a.wide) {
       tile_wide<Word, Idx>(a, f0, nf, lds);
       return;
@@ -1156,11 +1225,15 @@ void prepare_lds(const Op& op, char* data, const LaunchCfg& cfg, Args<Idx>& a, P
     a.vpr = vec ? (Idx)(op.R / V) : 1;
     a.st_per = vec ? (Idx)(op.C * a.vpr) : a.R * a.C * a.wu;
     const int64_t F = op.R * op.C * op.U;
-    if (tile_wide_ok(op) && 2 * tile_wide_copy(1, F) <= kWideLdsMax) {
-      // input order unless output order spreads the copies over more banks
-      // (input order: consecutive threads write R*U bytes apart; output
-      // order: they read C*U bytes apart)
-      a.wide = lds_banks_hit(op.C * op.U) > lds_banks_hit(op.R * op.U) ? 2 : 1;
+    if (tile_wide_ok(op) && tile_wide_copy(1, F) <= kWideLdsMax) {
+      // two LDS copies when they fit: input order unless output order
+      // spreads the copies over more banks (input order: consecutive threads
+      // write R*U bytes apart; output order: they read C*U bytes apart);
+      // else one copy, gathered from directly (wide 3)
+      if (2 * tile_wide_copy(1, F) <= kWideLdsMax)
+        a.wide = lds_banks_hit(op.C * op.U) > lds_banks_hit(op.R * op.U) ? 2 : 1;
+      else
+        a.wide = 3;
       a.cpr = 0;
       a.cps = 0;
       vec = false;
@@ -1223,15 +1296,16 @@ void prepare_lds(const Op& op, char* data, const LaunchCfg& cfg, Args<Idx>& a, P
   a.fld.init(a.ld_per);
   a.fst.init(a.st_per);
   a.fvpr.init(a.vpr);
-  p.K = pick_K((int64_t)a.nfp, (a.wide && op.kind == OP_TILE ? 2 : 1) * (int64_t)a.fpl,
-               cfg.lds_target);
+  // wide tiles: copies of the run of K matrices (two, or one for wide 3)
+  const int copies = (a.wide && op.kind == OP_TILE) ? (a.wide == 3 ? 1 : 2) : 0;
+  p.K = pick_K((int64_t)a.nfp, (copies ? copies : 1) * (int64_t)a.fpl, cfg.lds_target);
   if (a.wide && op.kind == OP_ROW)  // the run of K rows and its two partial chunks
     while (p.K > 1 && p.K * (int64_t)a.fpl + 32 > kWideLdsMax) --p.K;
-  if (a.wide && op.kind == OP_TILE)  // two copies of the run of K matrices
-    while (p.K > 1 && 2 * tile_wide_copy(p.K, a.fpl) > kWideLdsMax) --p.K;
+  if (copies)
+    while (p.K > 1 && copies * tile_wide_copy(p.K, a.fpl) > kWideLdsMax) --p.K;
   a.K = (Idx)p.K;
   p.lds = p.K * (int64_t)a.fpl + (a.wide && op.kind == OP_ROW ? 32 : 0);
-  if (a.wide && op.kind == OP_TILE) p.lds = 2 * tile_wide_copy(p.K, a.fpl);
+  if (copies) p.lds = copies * tile_wide_copy(p.K, a.fpl);
   p.threads = p.lds > 96 * 1024 ? cfg.threads_big : cfg.threads;
   p.groups = ((int64_t)a.nfp + p.K - 1) / p.K;
   p.variant = km * 10 + wl * 2 + (vec ? 1 : 0);
@@ -1960,9 +2034,10 @@ __device__ __forceinline__ void cycu_store(char* data, Off X, Off x0, Off x1, co
 // Word: the widest word dividing the cell and the tensor's address (below
 // 16 bytes); Off: int32_t when every offset of the pass (and of the
 // boundary area) stays below 2^31, else int64_t.
+// gtid: the thread's index among all threads of the pass (the lanes of an
+// item are consecutive threads of one wave)
 template <typename Word, typename Off, int D>
-__global__ __launch_bounds__(256) void k_cycle_u(CycArgs a, CycUArgs u) {
-  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void cycle_u_body(const CycArgs& a, const CycUArgs& u, const uint64_t gtid) {
   const int L = 1 << u.lanes_log2;
   const uint32_t lane = (uint32_t)(gtid & (uint64_t)(L - 1));
   const uint64_t item = gtid >> u.lanes_log2;
@@ -2072,41 +2147,57 @@ __global__ __launch_bounds__(256) void k_cycle_u(CycArgs a, CycUArgs u) {
   }
 }
 
+template <typename Word, typename Off, int D>
+__global__ __launch_bounds__(256) void k_cycle_u(CycArgs a, CycUArgs u) {
+  cycle_u_body<Word, Off, D>(a, u, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Item geometry of the shifted cycle kernel for cells of w-byte words: fills
+// u and a.items.  Lanes per item: G <= (lanes - 2) * 16 cell bytes need at
+// most lanes - 1 destination chunks, so the last lane only hands its chunk
+// down; cells larger than that are cut into equal groups of whole words (the
+// last one may be shorter; gmax is a multiple of 16, so G stays within it).
+// Returns whether every offset the kernel forms (within [-16, the end of the
+// pass or of the boundary area + 64 lanes' chunks + 32)) fits 32 bits.
+bool cycle_u_geometry(CycArgs& a, CycUArgs& u, uint64_t w) {
+  uint32_t ll = 2;
+  while (ll < 6 && ((uint64_t)((1u << ll) - 2) * 16) < a.U) ++ll;
+  u.lanes_log2 = ll;
+  const uint64_t gmax = (uint64_t)((1u << ll) - 2) * 16;
+  u.groups = (a.U + gmax - 1) / gmax;
+  u.G = ((a.U + u.groups - 1) / u.groups + w - 1) / w * w;
+  u.fgroups.init(u.groups);
+  a.items = a.B * a.nseg * u.groups;
+  const uint64_t span = std::max<uint64_t>(a.base + a.B * a.RCU, a.B * a.nslots * a.U) + 2048;
+  return span < (1ull << 31);
+}
+
 template <typename Word>
-hipError_t launch_cycle_u(CycArgs a, hipStream_t stream) {
+hipError_t launch_cycle_u(CycArgs a, int depth, hipStream_t stream) {
   if (a.nslots) {
     const uint64_t words = a.B * a.nslots * (a.U / sizeof(Word));
     const uint64_t blocks = std::min<uint64_t>((words + 255) / 256, 256ull * 64);
     hipLaunchKernelGGL((k_cycle_save<Word>), dim3((uint32_t)blocks), dim3(256), 0, stream, a);
   }
   CycUArgs u;
-  // lanes per item: G <= (lanes - 2) * 16 cell bytes need at most lanes - 1
-  // destination chunks, so the last lane only hands its chunk down
-  uint32_t ll = 2;
-  while (ll < 6 && ((uint64_t)((1u << ll) - 2) * 16) < a.U) ++ll;
-  u.lanes_log2 = ll;
-  const uint64_t gmax = (uint64_t)((1u << ll) - 2) * 16;
-  // cells larger than that are cut into equal groups of whole words (the
-  // last one may be shorter); gmax is a multiple of 16, so G stays within it
-  constexpr uint64_t w = sizeof(Word);
-  u.groups = (a.U + gmax - 1) / gmax;
-  u.G = ((a.U + u.groups - 1) / u.groups + w - 1) / w * w;
-  u.fgroups.init(u.groups);
-  a.items = a.B * a.nseg * u.groups;
-  const uint64_t blocks = ((a.items << ll) + 255) / 256;
+  const bool off32 = cycle_u_geometry(a, u, sizeof(Word));
+  const uint64_t blocks = ((a.items << u.lanes_log2) + 255) / 256;
   if (blocks >= (1ull << 31)) return hipErrorInvalidValue;
-  // every offset the kernel forms lies within [-16, end of the pass +
-  // 64 lanes' chunks + 32)
-  const uint64_t span = std::max<uint64_t>(a.base + a.B * a.RCU, a.B * a.nslots * a.U) + 2048;
-  if (span < (1ull << 31))
-    hipLaunchKernelGGL((k_cycle_u<Word, int32_t, 8>), dim3((uint32_t)blocks), dim3(256), 0, stream, a, u);
-  else
-    hipLaunchKernelGGL((k_cycle_u<Word, int64_t, 8>), dim3((uint32_t)blocks), dim3(256), 0, stream, a, u);
+  const dim3 grid((uint32_t)blocks), block(256);
+  if (off32) {
+    if (depth == 4) hipLaunchKernelGGL((k_cycle_u<Word, int32_t, 4>), grid, block, 0, stream, a, u);
+    else if (depth == 6) hipLaunchKernelGGL((k_cycle_u<Word, int32_t, 6>), grid, block, 0, stream, a, u);
+    else hipLaunchKernelGGL((k_cycle_u<Word, int32_t, 8>), grid, block, 0, stream, a, u);
+  } else {
+    if (depth == 4) hipLaunchKernelGGL((k_cycle_u<Word, int64_t, 4>), grid, block, 0, stream, a, u);
+    else if (depth == 6) hipLaunchKernelGGL((k_cycle_u<Word, int64_t, 6>), grid, block, 0, stream, a, u);
+    else hipLaunchKernelGGL((k_cycle_u<Word, int64_t, 8>), grid, block, 0, stream, a, u);
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_cycle(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
-                        int variant, hipStream_t stream, const void* table) {
+                        int variant, int depth, hipStream_t stream, const void* table) {
   CycArgs a;
   int64_t w = 1;
   bool empty = false;
@@ -2119,9 +2210,9 @@ hipError_t launch_cycle(const Op& op, char* data, char* scratch, int64_t scratch
       // VGPRs: 5 waves per SIMD instead of 4); its edge pieces still include
       // aligned 8-byte stores
       case 8:
-      case 4: return launch_cycle_u<uint32_t>(a, stream);
-      case 2: return launch_cycle_u<uint16_t>(a, stream);
-      default: return launch_cycle_u<uint8_t>(a, stream);
+      case 4: return launch_cycle_u<uint32_t>(a, depth, stream);
+      case 2: return launch_cycle_u<uint16_t>(a, depth, stream);
+      default: return launch_cycle_u<uint8_t>(a, depth, stream);
     }
   }
   switch (w) {
@@ -2940,9 +3031,17 @@ hipError_t launch_segtile(const Op& op, char* data, const LaunchCfg& cfg, hipStr
 //   80 + log2(word)*2 + vec     fused block permutation (item: K blocks)
 //   90 + log2(word)             boundary save of a cycle pass (item: blockDim words)
 //   100 + log2(word)*5 + log2(V) segmented tile (item: K matrices)
+//   120 + log2(word)            cycle following of cells that are not whole
+//                               16-byte words, in shifted 16-byte chunks
+//                               (item: blockDim threads; 32-bit offsets)
+struct CycUOp {
+  CycArgs a;
+  CycUArgs u;
+};
 constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
 constexpr size_t kBatchArgBytes =
-    cmax(cmax(sizeof(Args<uint32_t>), sizeof(CycArgs)), cmax(sizeof(BlkArgs), sizeof(SegArgs)));
+    cmax(cmax(cmax(sizeof(Args<uint32_t>), sizeof(CycArgs)), cmax(sizeof(BlkArgs), sizeof(SegArgs))),
+         sizeof(CycUOp));
 struct BatchOpDev {
   int variant;
   int pad[3];
@@ -2988,6 +3087,13 @@ __device__ __forceinline__ void batch_body(const BatchOpDev& op, uint32_t f0, un
     cycle_save_word<typename WordOf<WL>::T>(*reinterpret_cast<const CycArgs*>(op.raw),       \
                                             (uint64_t)item.f0 * blockDim.x + threadIdx.x);   \
     break;
+#define EH_CYCU(WL)                                                                           \
+  case 120 + WL: {                                                                            \
+    const CycUOp& c = *reinterpret_cast<const CycUOp*>(op.raw);                               \
+    cycle_u_body<typename WordOf<WL>::T, int32_t, 4>(c.a, c.u,                                \
+                                                    (uint64_t)item.f0 * blockDim.x + threadIdx.x); \
+    break;                                                                                    \
+  }
 #define EH_BLK(WL, VF)                                                                      \
   case 80 + WL * 2 + VF:                                                                    \
     block_body<typename WordOf<WL>::T, VF ? (16 >> WL) : 1>(                                \
@@ -3065,10 +3171,14 @@ __global__ __launch_bounds__(512) void k_batch_cycles(const BatchOpDev* __restri
       EH_CYC(2)
       EH_CYC(3)
       EH_CYC(4)
+      EH_CYCU(0)
+      EH_CYCU(1)
+      EH_CYCU(2)
       default: break;
     }
   }
 }
+#undef EH_CYCU
 #undef EH_BLK
 #undef EH_SEG
 #undef EH_CYC
@@ -3088,7 +3198,8 @@ hipError_t prepare_op(const Op& op) {
 hipError_t launch_op(const Op& op, char* data, char* scratch, int64_t scratch_bytes,
                      const LaunchCfg& cfg, hipStream_t stream, const void* table) {
   if (op.kind == OP_CYCLE)
-    return launch_cycle(op, data, scratch, scratch_bytes, cfg.cycle_variant, stream, table);
+    return launch_cycle(op, data, scratch, scratch_bytes, cfg.cycle_variant, cfg.cycle_u_depth, stream,
+                        table);
   if (op.kind == OP_BLOCK) return launch_block(op, data, cfg, stream, table);
   if (op.kind == OP_SEGTILE) return launch_segtile(op, data, cfg, stream);
   if (op.kind == 0 || op.us <= 0 || op.U % op.us) return hipErrorInvalidValue;
@@ -3182,12 +3293,28 @@ int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, cons
       }
       if (empty) continue;
       boff += need;
-      std::memcpy(d.raw, &a, sizeof(a));
-      d.variant = 70 + log2i(w);
+      // cells that are not whole 16-byte words move in shifted 16-byte
+      // chunks (cells of 8-byte words as of 4-byte words, as in single
+      // launches) when every offset fits 32 bits, else word by word
+      CycUOp cu;
+      cu.a = a;
+      const int64_t wu = w == 8 ? 4 : w;
+      const bool shifted = w < 16 && cfg.cycle_variant == 0 && cycle_u_geometry(cu.a, cu.u, (uint64_t)wu);
+      uint64_t groups;
+      if (shifted) {
+        std::memcpy(d.raw, &cu, sizeof(cu));
+        d.variant = 120 + log2i(wu);
+        groups = ((cu.a.items << cu.u.lanes_log2) + threads - 1) / threads;
+      } else {
+        std::memcpy(d.raw, &a, sizeof(a));
+        d.variant = 70 + log2i(w);
+        groups = ((a.items << a.lane_log2) + threads - 1) / threads;
+      }
       const uint32_t idx = (uint32_t)hops.size();
       hops.push_back(d);
-      const uint64_t groups = ((a.items << a.lane_log2) + threads - 1) / threads;
       for (uint64_t g = 0; g < groups; ++g) cycles.push_back(BatchItemDev{idx, (uint32_t)g});
+      // the boundary save below reads the plain cycle arguments
+      std::memcpy(d.raw, &a, sizeof(a));
       if (a.nslots) {
         d.variant = 90 + log2i(w);
         const uint32_t sidx = (uint32_t)hops.size();

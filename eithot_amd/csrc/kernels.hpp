@@ -22,6 +22,7 @@ struct LaunchCfg {
   int segtile_threads = 0;         // > 0: workgroup size of segmented tile passes
   int lds_threads_auto = 1;        // 1: tile/column/row workgroups sized for one load round
   int segtile_nt = 3;              // non-temporal segmented tile loads (bit 0) and stores (bit 1)
+  int cycle_u_depth = 4;           // positions in flight per item of the shifted cycle kernel
 };
 
 // One pass of `op` over the tensor at `data`, on `stream`.  Staged ops use

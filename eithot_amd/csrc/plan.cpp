@@ -920,6 +920,7 @@ static int64_t* tuning_field(Tuning& t, const char* key) {
   if (!strcmp(key, "segtile_nt")) return &t.segtile_nt;
   if (!strcmp(key, "lower_mask")) return &t.lower_mask;
   if (!strcmp(key, "hold_floor")) return &t.hold_floor;
+  if (!strcmp(key, "cycle_u_depth")) return &t.cycle_u_depth;
   return nullptr;
 }
 
@@ -937,6 +938,7 @@ int tuning_set(const char* key, int64_t value) {
   if (f == &t.lds_target && value < 1024) return EITHOT_ERR_INVALID;
   if (f == &t.segtile_nt && value != 0 && value != 2 && value != 3) return EITHOT_ERR_INVALID;
   if (f == &t.lower_mask && (value == 0 || value > LOWER_ALL)) return EITHOT_ERR_INVALID;
+  if (f == &t.cycle_u_depth && value != 4 && value != 6 && value != 8) return EITHOT_ERR_INVALID;
   const bool flag = f == &t.regroup || f == &t.lds_threads_auto || f == &t.cycle_cell_pref ||
                     f == &t.cycle_word_cost || f == &t.block_digits || f == &t.xcd_remap;
   if (flag && value > 1) return EITHOT_ERR_INVALID;

@@ -95,6 +95,7 @@ LaunchCfg Device::launch_cfg() const {
   c.segtile_threads = (int)t.segtile_threads;
   c.lds_threads_auto = (int)t.lds_threads_auto;
   c.segtile_nt = (int)t.segtile_nt;
+  c.cycle_u_depth = (int)t.cycle_u_depth;
   return c;
 }
 

@@ -212,6 +212,8 @@ int eithot_plan_segments(const int64_t* dims, const int* perm, int order, size_t
  *                    for cells of at least 64 words)
  *   "cycle_lmax"     > 0: cycle segments of at most this many positions
  *                    (0: chosen per pass)
+ *   "cycle_u_depth"  positions in flight per item when cycle following moves
+ *                    cells that are not whole 16-byte words (4 default, 6, 8)
  *   "cycle_cell_pref" cycle passes of cells of 256 B and more cost less in
  *                    the planner (0 default: not faster on MI355X so far)
  *   "cycle_word_cost" cycle passes over cells that are not whole 16-byte

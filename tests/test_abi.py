@@ -50,10 +50,11 @@ def test_error_message_thread_local_and_planning_without_gpu():
 def test_tuning_rejects_out_of_range_values():
     import eithot_amd as E
     for key, bad in (("segtile_nt", 1), ("regroup", 2), ("lds_threads_auto", 5),
-                     ("cycle_variant", 8), ("threads", 96), ("cycle_lmax", 1)):
+                     ("cycle_variant", 8), ("threads", 96), ("cycle_lmax", 1),
+                     ("cycle_u_depth", 5), ("cycle_u_depth", 16)):
         with pytest.raises(E.EithotError):
             E.set_tuning(key, bad)
-    for key, good in (("segtile_nt", 2), ("regroup", 0), ("cycle_lmax", 64)):
+    for key, good in (("segtile_nt", 2), ("regroup", 0), ("cycle_lmax", 64), ("cycle_u_depth", 8)):
         old = E.set_tuning(key, good)
         try:
             assert E.get_tuning(key) == good
