@@ -208,8 +208,11 @@ int eithot_plan_segments(const int64_t* dims, const int* perm, int order, size_t
  *   "fuse_max_block" largest block of a fused tile permutation (65536; 0 = off)
  *   "block_digits"   fused blocks compute source cells from digits (1) or read
  *                    a gather table (0)
- *   "cycle_variant"  cycle-following kernel shape (0..7; 0 default; 1..3 only
- *                    for cells of at least 64 words)
+ *   "cycle_variant"  cycle-following kernel shape (0..7; 1..3 only for cells of
+ *                    at least 64 words).  0 (default) picks: shifted 16-byte
+ *                    chunks for cells that are not whole 16-byte words,
+ *                    non-temporal loads and stores (as 5) for cells of at
+ *                    least 512 bytes, else plain 16-byte words
  *   "cycle_lmax"     > 0: cycle segments of at most this many positions
  *                    (0: chosen per pass)
  *   "cycle_u_depth"  positions in flight per item when cycle following moves
