@@ -360,3 +360,38 @@ def test_dense_block_passes(torch_cuda):
                 assert_bitexact(dev.cpu().numpy(), want, f"offset digits {digits} {dims} {perm}")
     finally:
         E.set_tuning("block_digits", 1)
+
+
+NARROW_TILE_CASES = [([10, 13403, 2], [2, 1, 3], 1),     # 134,030-byte matrices: one LDS copy
+                     ([11, 9222, 2], [2, 1, 3], 1),      # 101,442 bytes: one copy
+                     ([780, 139, 3], [2, 1, 3], 1),      # 108,420 bytes: one copy
+                     ([6, 770, 24, 3], [1, 3, 2, 4], 1),  # 6-byte units, 110,880 bytes: one copy
+                     ([2, 40732, 2], [2, 1, 3], 1),      # 81,464 bytes: two copies
+                     ([97, 416, 5], [2, 1, 3], 2),       # 2-byte units, 80,704 bytes: two copies
+                     ([4307, 16, 3], [2, 1, 3], 1),      # 68,912 bytes: two copies
+                     ([9, 283, 2, 40], [1, 3, 2, 4], 2),  # 18-byte units in 2 x 283 matrices
+                     ([64, 96, 5], [2, 1, 3], 1)]        # one-word cells that pack: not wide
+
+
+def test_narrow_word_tiles(torch_cuda):
+    # tile passes of 1- and 2-byte words: matrices run through LDS as aligned
+    # 16-byte chunks, transposed between two LDS copies or gathered from one
+    # copy through a digit odometer; at the tensor's own address and one
+    # element past it (every matrix then starts at another offset mod 16)
+    import eithot_amd as E
+    from oracle import pyoracle as O
+    for dims, perm, elem in NARROW_TILE_CASES:
+        ops, _ = E.plan_describe(dims, perm, elem, 1, 0.05)
+        tiles = [o for o in ops if o["kind"] == 1 and not o["staged"]]
+        assert tiles and all(o["us"] == o["U"] and (o["U"] % 4) for o in tiles), (dims, ops)
+        got, want, _ = run_case(torch_cuda, dims, perm, elem, seed=22, alpha=0.05)
+        assert_bitexact(got, want, f"narrow tile {dims} {perm} elem {elem}")
+        vol = int(np.prod(dims))
+        host = O.fill(vol + 1, elem, 23, 1)
+        dev = torch_cuda.from_numpy(host.copy()).to("cuda:0")
+        print(f"case offset 1 dims={dims} perm={perm} elem={elem}", flush=True)
+        E.transpose(dev, 1, len(dims), dims, [p - 1 for p in perm], elem, 1, 0.05)
+        torch_cuda.cuda.synchronize()
+        want = host.copy()
+        want[elem:] = O.permute(np.ascontiguousarray(host[elem:]), dims, [p - 1 for p in perm], elem)
+        assert_bitexact(dev.cpu().numpy(), want, f"narrow tile offset {dims} {perm} elem {elem}")
