@@ -1118,9 +1118,14 @@ bool needs64(const Op& op) {
   if (big((unsigned __int128)op.num_footprints() * 2)) return true;
   if (big((unsigned __int128)op.footprint_units() * op.us * 2)) return true;
   if (op.kind == OP_TILE) return big((unsigned __int128)op.R * op.C * 2);
-  const unsigned __int128 m = op.m, n = op.n, a = op.a, b = op.b, c = op.c;
-  return big(m * m + n) || big(b * b) || big(a * a) || big(a * b) || big(n + 2 * m) ||
-         big(c * b + m) || big(m * 3);
+  const unsigned __int128 m = op.m, n = op.n, a = op.a;
+  if (big(n + 2 * m) || big(m * 3)) return true;
+  // the products each map forms (maps.hpp): rot/rotinv i + j/b; colsh
+  // i * (n mod m) + j; colshinv a * a and a * c; rowg q * b < n and
+  // (j/c) * k < b * b, rowd c * rho + t < n + m and r * a < a * b, the two
+  // products reduced mod b in 64 bits whatever the index width
+  if (op.kind == OP_COL) return big(m * m + n) || big(a * a);
+  return false;
 }
 
 int64_t word_bytes(const Op& op, uintptr_t data) {

@@ -143,6 +143,14 @@ struct GridMaps {
   using FD = typename FDsel<Idx>::type;
   Idx m = 1, n = 1, c = 1, a = 1, b = 1, k = 0, binv = 0, nm = 0;  // nm = n mod m
   FD fm, fc, fa, fb;
+  // b as a 64-bit divisor: the row maps' products rho * k < b * b and
+  // r * a < a * b are reduced mod b in 64 bits even when Idx is 32-bit
+  FD64 fb64;
+  // x * y mod b for x, y < 2^32 (exact in 64 bits)
+  EH_HD Idx mulmod_b(Idx x, Idx y) const {
+    if constexpr (sizeof(Idx) == 4) return (Idx)fb64.mod((uint64_t)x * (uint64_t)y);
+    else return fb.mod(x * y);
+  }
 
   void init(uint64_t m_, uint64_t n_, uint64_t c_, uint64_t a_, uint64_t b_, uint64_t k_,
             uint64_t binv_) {
@@ -158,6 +166,7 @@ struct GridMaps {
     fc.init(c_);
     fa.init(a_);
     fb.init(b_);
+    fb64.init(b_);
   }
 
   EH_HD Idx rot(Idx i, Idx j) const {
@@ -191,13 +200,13 @@ struct GridMaps {
     if (t >= m) t -= m;
     Idx y = j >= t ? j - t : j + n - t;
     Idx rho = fc.div(y);
-    Idx r = fb.mod(rho * k);
+    Idx r = mulmod_b(rho, k);
     return q * b + r;
   }
   EH_HD Idx rowd(Idx i, Idx j) const {
     Idx q = fb.div(j);
     Idx r = j - q * b;
-    Idx rho = fb.mod(r * a);
+    Idx rho = mulmod_b(r, a);
     Idx t = i + q;
     if (t >= m) t -= m;
     Idx y = c * rho + t;

@@ -50,7 +50,10 @@ def consts(m, n):
 
 
 SHAPES = [(m, n) for m in (1, 2, 3, 4, 6, 8, 12, 17, 32) for n in (1, 2, 5, 8, 12, 18, 32, 45)
-          if m <= n] + [(1024, 4096), (64, 11520), (32, 552960), (65536, 163840)]
+          if m <= n] + [(1024, 4096), (64, 11520), (32, 552960), (65536, 163840),
+                        # prime row lengths of the served workload: b * b and a * b far
+                        # beyond 32 bits, reduced mod b in 64 bits by 32-bit maps
+                        (20, 16060853), (141, 602891), (56, 2578133), (48, 2147483)]
 
 
 @pytest.mark.parametrize("m,n", SHAPES)
@@ -61,17 +64,22 @@ def test_maps_equal_reference_functors(mh, m, n):
         pts = [(i, j) for i in range(m) for j in range(n)]
     else:
         pts = [(int(i), int(j)) for i, j in zip(rng.integers(0, m, 3000), rng.integers(0, n, 3000))]
-    wides = (0, 1) if m * m + n < (1 << 31) and b * b < (1 << 31) else (1,)
-    for wide in wides:
+    # 32-bit maps are used where the planner allows them: column maps while
+    # m*m + n and a*a stay below 2^31, row maps while n + 2m does
+    col32 = m * m + n < (1 << 31) and a * a < (1 << 31)
+    row32 = n + 2 * m < (1 << 31)
+    for wide in (0, 1):
         for i, j in pts:
             ev = lambda w: mh.mh_eval(w, wide, m, n, c, a, b, k, binv, i, j)  # noqa: E731
-            assert ev(1) == O.fn_c2r_rotate(m, b, i, j)
-            assert ev(2) == O.fn_r2c_rotate(m, b, i, j)
-            assert ev(3) == O.fn_c2r_col_shuffle(m, n, c, i, j)
-            assert ev(4) == O.fn_r2c_col_shuffle(a, c, m, binv, i, j)
-            assert ev(5) == O.fn_c2r_row_shuffle(m, n, c, k, i, j)
-            # rowd is the inverse of the reference's row shuffle gather
-            assert O.fn_c2r_row_shuffle(m, n, c, k, i, ev(6)) == j
+            if wide or col32:
+                assert ev(1) == O.fn_c2r_rotate(m, b, i, j)
+                assert ev(2) == O.fn_r2c_rotate(m, b, i, j)
+                assert ev(3) == O.fn_c2r_col_shuffle(m, n, c, i, j)
+                assert ev(4) == O.fn_r2c_col_shuffle(a, c, m, binv, i, j)
+            if wide or row32:
+                assert ev(5) == O.fn_c2r_row_shuffle(m, n, c, k, i, j)
+                # rowd is the inverse of the reference's row shuffle gather
+                assert O.fn_c2r_row_shuffle(m, n, c, k, i, ev(6)) == j
 
 
 def test_magic_division(mh):
