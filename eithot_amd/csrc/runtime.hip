@@ -183,7 +183,7 @@ int Device::reserve_arena_locked(int64_t bytes, hipStream_t stream) {
     // every earlier user is done once its event has completed (users
     // enqueue under the same lock)
     if (arena_ready) (void)hipEventSynchronize(arena_ready);
-    hipFree(arena);
+    if (hipFree(arena) != hipSuccess) (void)hipGetLastError();
     arena = nullptr;
     arena_bytes = 0;
   }
@@ -224,9 +224,8 @@ int run_plan(Device* d, const Plan& plan, char* data, char* scratch, int64_t scr
   std::vector<hipEvent_t> ev;
   if (times) {
     const std::vector<hipEvent_t>& pool = timing_events(plan.ops.size() + 1);
-    if (pool.size() >= plan.ops.size() + 1) {
+    if (pool.size() >= plan.ops.size() + 1 && hipEventRecord(pool[0], stream) == hipSuccess) {
       ev.assign(pool.begin(), pool.begin() + (long)(plan.ops.size() + 1));
-      hipEventRecord(ev[0], stream);
     } else {
       times->clear();
       times = nullptr;  // no events to time with
@@ -240,15 +239,22 @@ int run_plan(Device* d, const Plan& plan, char* data, char* scratch, int64_t scr
       rc = fail(EITHOT_ERR_DEVICE, "pass %zu launch failed: %s", i, hipGetErrorString(e));
       break;
     }
-    if (times) hipEventRecord(ev[i + 1], stream);
+    if (times && hipEventRecord(ev[i + 1], stream) != hipSuccess) {
+      rc = fail(EITHOT_ERR_DEVICE, "pass %zu timing event failed", i);
+      break;
+    }
   }
   if (times) {
     times->clear();
     if (rc == EITHOT_OK) {
-      hipEventSynchronize(ev.back());
+      if (hipEventSynchronize(ev.back()) != hipSuccess)
+        return fail(EITHOT_ERR_DEVICE, "pass timing synchronisation failed");
       for (size_t i = 0; i < plan.ops.size(); ++i) {
         float ms = 0;
-        hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+        if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess) {
+          times->clear();
+          return fail(EITHOT_ERR_DEVICE, "pass %zu elapsed time unavailable", i);
+        }
         times->push_back(ms);
       }
     }
@@ -584,12 +590,14 @@ int eithot_verify_pattern(const void* dev, const int64_t* dims, const int* perm,
   unsigned long long* d_cnt = nullptr;
   if (hipMalloc((void**)&d_cnt, sizeof(unsigned long long)) != hipSuccess)
     return fail(EITHOT_ERR_NOMEM, "counter allocation failed");
-  hipMemset(d_cnt, 0, sizeof(unsigned long long));
-  hipError_t e = launch_verify((const char*)dev, dims, p0.data(), order, (int)elem_bytes, seed,
-                               mode, d_cnt, nullptr);
+  hipError_t e = hipMemset(d_cnt, 0, sizeof(unsigned long long));
+  if (e == hipSuccess)
+    e = launch_verify((const char*)dev, dims, p0.data(), order, (int)elem_bytes, seed, mode,
+                      d_cnt, nullptr);
   unsigned long long h = 0;
   if (e == hipSuccess) e = hipMemcpy(&h, d_cnt, sizeof(h), hipMemcpyDeviceToHost);
-  hipFree(d_cnt);
+  const hipError_t fe = hipFree(d_cnt);
+  if (e == hipSuccess) e = fe;
   if (e != hipSuccess) return fail(EITHOT_ERR_DEVICE, "verify failed: %s", hipGetErrorString(e));
   if (mismatches) *mismatches = h;
   return EITHOT_OK;
