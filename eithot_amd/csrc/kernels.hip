@@ -989,6 +989,126 @@ __global__ __launch_bounds__(256) void k_stage_in_row(Args<Idx> a) {
   }
 }
 
+// Staged row pass of whole units (S == 1), part 2, tiled through LDS.  The
+// row-gather map has rowg(i, x + m) = rowg(i, x) + 1: units m apart map to
+// consecutive units.  A tile is the index range [t0, t0 + T), T = m * L;
+// rowg sends x = t0 + v + m*u (u < L) to L consecutive units for each v, so
+// lanes ordered (v, u, word) move runs of L units on the rowg side:
+//  - MAP_ROWG (unit x of the row <- scratch unit rowg(i, x)): word loads
+//    from scratch in runs, LDS in row order, aligned 16-byte stores of the
+//    tile's contiguous bytes;
+//  - MAP_ROWD (unit x <- scratch unit rowd(i, x); rowd inverts rowg, so
+//    scratch unit x goes to unit rowg(i, x)): aligned 16-byte loads of the
+//    tile's contiguous scratch bytes, word stores in runs.
+// Each unit's rowg is evaluated once, into LDS, so the result never depends
+// on the runs being unbroken; grid.y strides the group's rows.
+template <typename Word, typename Idx, int MAP>
+__global__ __launch_bounds__(256) void k_stage_row_tile(Args<Idx> a, Idx L,
+                                                        typename FDsel<Idx>::type fL) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  constexpr Idx w = sizeof(Word);
+  const Idx tid = threadIdx.x, nt = blockDim.x;
+  const Idx U = (Idx)a.U, m = a.g.m, n = a.C, T = m * L, wpu = a.wu;
+  Idx* gi = reinterpret_cast<Idx*>(lds);
+  unsigned char* buf = lds + ((T * (Idx)sizeof(Idx) + 15) & ~(Idx)15);
+  const uint64_t ntile = ((uint64_t)n + T - 1) / T;
+  const uintptr_t d0 = (uintptr_t)a.data, s0 = (uintptr_t)a.scratch;
+  const int64_t sc_end = (int64_t)(a.G * a.CU);
+  for (uint64_t fl = blockIdx.y; fl < a.G; fl += gridDim.y) {
+    const Foot<Idx, OP_ROW> f = decode<Idx, OP_ROW>(a, (Idx)(a.fp0 + fl));
+    const int64_t sro = (int64_t)(fl * a.CU);
+    for (uint64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+      const Idx t0 = (Idx)(tile * T);
+      const Idx tn = n - t0 < T ? n - t0 : T;
+      const Idx nw = T * wpu;  // (v, u, word) items, those with v + m*u >= tn idle
+      __syncthreads();         // the previous tile's LDS reads are done
+      for (Idx e = tid; e < tn; e += nt) gi[e] = a.g.rowg(f.row, t0 + e);
+      if (MAP == MAP_ROWG) {
+        const int64_t g0 = (int64_t)f.addr + (int64_t)t0 * U, g1 = g0 + (int64_t)tn * U;
+        const Idx sh = (Idx)((d0 + (uint64_t)g0) & 15);
+        __syncthreads();
+        for (Idx q0 = tid; q0 < nw; q0 += nt * UNR) {
+          Word v[UNR];
+          Idx e[UNR], xw[UNR];
+#pragma unroll
+          for (int k = 0; k < UNR; ++k) {
+            const Idx q = q0 + (Idx)k * nt;
+            e[k] = tn;
+            if (q >= nw) continue;
+            const Idx r = a.fwu.div(q);
+            xw[k] = q - r * wpu;
+            const Idx vv = fL.div(r);
+            e[k] = vv + m * (r - vv * L);
+            if (e[k] < tn)
+              v[k] = *reinterpret_cast<const Word*>(a.scratch + sro + (int64_t)gi[e[k]] * U + xw[k] * w);
+          }
+#pragma unroll
+          for (int k = 0; k < UNR; ++k)
+            if (e[k] < tn) *reinterpret_cast<Word*>(buf + sh + e[k] * U + xw[k] * w) = v[k];
+        }
+        __syncthreads();
+        const int64_t a0 = g0 - sh;
+        const Idx nch = (Idx)((g1 - a0 + 15) / 16);
+        for (Idx t = tid; t < nch; t += nt) {
+          const int64_t cs = a0 + 16 * (int64_t)t;
+          if (cs >= g0 && cs + 16 <= g1) {
+            *reinterpret_cast<W16*>(a.data + cs) = *reinterpret_cast<const W16*>(buf + t * 16);
+          } else {
+            const int64_t b0 = cs < g0 ? g0 : cs, b1 = cs + 16 < g1 ? cs + 16 : g1;
+            for (int64_t b = b0; b < b1; b += w)
+              *reinterpret_cast<Word*>(a.data + b) = *reinterpret_cast<const Word*>(buf + (b - a0));
+          }
+        }
+      } else {
+        // scratch bytes of the tile, [g0, g1) from scratch, at their address mod 16
+        const int64_t g0 = sro + (int64_t)t0 * U, g1 = g0 + (int64_t)tn * U;
+        const Idx sh = (Idx)((s0 + (uint64_t)g0) & 15);
+        const int64_t a0 = g0 - sh;
+        const Idx nch = (Idx)((g1 - a0 + 15) / 16);
+        for (Idx t = tid; t < nch; t += nt) {
+          const int64_t cs = a0 + 16 * (int64_t)t;
+          if (cs >= 0 && cs + 16 <= sc_end) {
+            *reinterpret_cast<W16*>(buf + t * 16) = *reinterpret_cast<const W16*>(a.scratch + cs);
+          } else {
+            const int64_t b0 = cs < g0 ? g0 : cs, b1 = cs + 16 < g1 ? cs + 16 : g1;
+            for (int64_t b = b0; b < b1; b += w)
+              *reinterpret_cast<Word*>(buf + (b - a0)) = *reinterpret_cast<const Word*>(a.scratch + b);
+          }
+        }
+        __syncthreads();
+        for (Idx q = tid; q < nw; q += nt) {
+          const Idx r = a.fwu.div(q);
+          const Idx xw = q - r * wpu;
+          const Idx vv = fL.div(r);
+          const Idx e = vv + m * (r - vv * L);
+          if (e < tn)
+            *reinterpret_cast<Word*>(a.data + f.addr + (uint64_t)gi[e] * U + xw * w) =
+                *reinterpret_cast<const Word*>(buf + sh + e * U + xw * w);
+        }
+      }
+    }
+  }
+}
+
+// LDS bytes of k_stage_row_tile for runs of L units: the indices, then the
+// tile's bytes from their address mod 16 through the end of the last 16-byte
+// chunk that holds any of them (whole chunks are moved)
+inline int64_t stage_row_tile_lds(int64_t m, int64_t L, int64_t U, int64_t idx_bytes) {
+  const int64_t T = m * L;
+  return ((T * idx_bytes + 15) & ~(int64_t)15) + ((T * U + 30) & ~(int64_t)15);
+}
+
+// Run length of k_stage_row_tile for a staged row pass: tiles of about
+// 16 KiB of LDS (units and their indices) and runs of at least 96 bytes,
+// within 48 KiB; 0 when no such tile fits (the pass then gathers unit by
+// unit).
+inline int64_t stage_row_tile_len(const Op& op, int64_t idx_bytes) {
+  if (op.kind != OP_ROW || op.m < 1 || op.U < 1) return 0;
+  int64_t L = std::max<int64_t>(16384 / (op.m * (op.U + idx_bytes)), (96 + op.U - 1) / op.U);
+  if (L > op.C) L = op.C;
+  return stage_row_tile_lds(op.m, L, op.U, idx_bytes) <= 48 * 1024 ? L : 0;
+}
+
 // staged tile, part 2: out-of-place LDS-tiled transpose of the footprints
 // held in scratch (each an R x C row-major matrix of us-byte cells) back into
 // the tensor, where output cell (j, i) of footprint f lives at
@@ -1386,6 +1506,9 @@ struct LdsLauncher {
   }
 };
 
+hipError_t launch_copy16(char* dst, const char* src, uint64_t len, const char* slo, const char* shi,
+                         hipStream_t st);
+
 template <typename Word, typename Idx, int KIND, int MAP>
 hipError_t launch_staged(Args<Idx> a, const Op& op, char* scratch, int64_t scratch_bytes,
                          hipStream_t st) {
@@ -1403,8 +1526,8 @@ hipError_t launch_staged(Args<Idx> a, const Op& op, char* scratch, int64_t scrat
     if ((KIND == OP_TILE || KIND == OP_ROW) && a.S == 1) {
       // whole matrices, or whole rows: one contiguous range
       const uint64_t fpb_all = KIND == OP_TILE ? a.RCU : a.CU;
-      hipError_t e = hipMemcpyAsync(scratch, a.data + a.base + f0 * fpb_all, a.G * fpb_all,
-                                    hipMemcpyDeviceToDevice, st);
+      hipError_t e = launch_copy16(scratch, a.data + a.base + f0 * fpb_all, a.G * fpb_all,
+                                   a.data + op.base, a.data + op.base + op.bytes(), st);
       if (e != hipSuccess) return e;
     } else {
       hipLaunchKernelGGL((k_stage_out<Word, Idx, KIND>), dim3((uint32_t)blocks), dim3(256), 0, st, a);
@@ -1440,6 +1563,16 @@ hipError_t launch_staged(Args<Idx> a, const Op& op, char* scratch, int64_t scrat
         hipLaunchKernelGGL((k_tile_oop<Word, Idx, V>), dim3(gx, gy), dim3(256), lds, st, a, tg);
       else
         hipLaunchKernelGGL((k_tile_oop<Word, Idx, 1>), dim3(gx, gy), dim3(256), lds, st, a, tg);
+    } else if (KIND == OP_ROW && a.S == 1 && stage_row_tile_len(op, sizeof(Idx)) > 0) {
+      const int64_t L = stage_row_tile_len(op, sizeof(Idx));
+      const uint64_t ntile = ((uint64_t)op.C + op.m * L - 1) / (uint64_t)(op.m * L);
+      const uint32_t gx = (uint32_t)std::min<uint64_t>(ntile, 1u << 20);
+      const uint32_t gy = (uint32_t)std::min<uint64_t>(a.G, 65535);
+      typename FDsel<Idx>::type fL;
+      fL.init((uint64_t)L);
+      hipLaunchKernelGGL((k_stage_row_tile<Word, Idx, (KIND == OP_ROW ? MAP : MAP_ROWG)>), dim3(gx, gy),
+                         dim3(256), (size_t)stage_row_tile_lds(op.m, L, op.U, sizeof(Idx)), st, a,
+                         (Idx)L, fL);
     } else if (KIND == OP_ROW && a.S == 1) {
       const uint64_t nch = (a.CU + 31) / 16;  // a row's 16-byte chunks, edges included
       const uint32_t gx = (uint32_t)std::min<uint64_t>((nch + 255) / 256, 1024);
@@ -1975,6 +2108,62 @@ __device__ __forceinline__ W16 funnel16(const W16& a, const W16& b, uint32_t sig
   out.z = __builtin_amdgcn_alignbyte(f3, f2, r);
   out.w = __builtin_amdgcn_alignbyte(f4, f3, r);
   return out;
+}
+
+// Device copy of len bytes from src to dst (ranges apart), in aligned
+// 16-byte stores: output chunk t holds dst bytes [D0 + 16t, D0 + 16t + 16),
+// D0 = dst rounded down to 16, taken as bytes [sig, sig + 16) of the aligned
+// source chunks at S0 + 16t and S0 + 16t + 16, where S0 + sig is the source
+// byte of output byte D0.  Source chunks are loaded whole only within
+// [slo, shi), bytes known to exist; output chunks at the range's ends, and
+// any whose source chunks leave [slo, shi), move byte by byte.
+__global__ __launch_bounds__(256) void k_copy16(char* dst, const char* src, uint64_t len,
+                                                const char* slo, const char* shi) {
+  constexpr int CU = 4;  // chunks per thread per step, loads issued together
+  const int64_t D = (int64_t)(uintptr_t)dst, end = D + (int64_t)len;
+  const int64_t D0 = D & ~(int64_t)15;
+  const int64_t S = (int64_t)(uintptr_t)src - (D - D0);
+  const int64_t S0 = S & ~(int64_t)15;
+  const uint32_t sig = (uint32_t)(S - S0);
+  const int64_t lo = (int64_t)(uintptr_t)slo, hi = (int64_t)(uintptr_t)shi;
+  const uint64_t nch = (uint64_t)((end - D0 + 15) / 16);
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t0 < nch; t0 += stride * CU) {
+    W16 c[CU], nx[CU];
+    bool whole[CU];
+#pragma unroll
+    for (int u = 0; u < CU; ++u) {
+      const uint64_t t = t0 + (uint64_t)u * stride;
+      const int64_t oc = D0 + 16 * (int64_t)t, sc = S0 + 16 * (int64_t)t;
+      whole[u] = t < nch && oc >= D && oc + 16 <= end && sc >= lo && sc + 32 <= hi;
+      if (whole[u]) {
+        c[u] = *reinterpret_cast<const W16*>(sc);
+        if (sig) nx[u] = *reinterpret_cast<const W16*>(sc + 16);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < CU; ++u) {
+      const uint64_t t = t0 + (uint64_t)u * stride;
+      if (t >= nch) continue;
+      const int64_t oc = D0 + 16 * (int64_t)t;
+      if (whole[u]) {
+        *reinterpret_cast<W16*>(oc) = sig ? funnel16(c[u], nx[u], sig) : c[u];
+      } else {
+        const int64_t b0 = oc < D ? D : oc, b1 = oc + 16 < end ? oc + 16 : end;
+        for (int64_t b = b0; b < b1; ++b)
+          *reinterpret_cast<char*>(b) = *reinterpret_cast<const char*>(b - D + (int64_t)(uintptr_t)src);
+      }
+    }
+  }
+}
+
+hipError_t launch_copy16(char* dst, const char* src, uint64_t len, const char* slo, const char* shi,
+                         hipStream_t st) {
+  if (!len) return hipSuccess;
+  const uint64_t nch = (len + 31) / 16;
+  const uint64_t blocks = std::min<uint64_t>((nch + 1023) / 1024, 256ull * 64);
+  hipLaunchKernelGGL(k_copy16, dim3((uint32_t)blocks), dim3(256), 0, st, dst, src, len, slo, shi);
+  return hipGetLastError();
 }
 
 // 16 bytes from offset x of base, only those within [lo, hi) read (the

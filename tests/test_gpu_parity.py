@@ -76,6 +76,51 @@ def test_staged_passes(torch_cuda):
         assert peak <= 1.0 * got.nbytes
 
 
+# Staged Catanzaro row passes whose rows exceed LDS, tiled through LDS in
+# runs of units (m = grid rows, U = unit bytes): row gather (map 5) and its
+# inverse (map 6), words of 1 to 16 bytes, groups of one row and of several
+STAGED_ROW_CASES = [
+    ([6, 100003, 20], [1, 3, 2], 4, 0.05),   # m 20, 24-byte units, one row per group
+    ([6, 20, 100003], [1, 3, 2], 4, 0.05),
+    ([15, 30011, 141], [1, 3, 2], 2, 0.05),  # m 141, 30-byte units (2-byte words)
+    ([15, 141, 30011], [1, 3, 2], 2, 0.05),
+    ([2, 40009, 7], [1, 3, 2], 8, 0.3),      # 16-byte units, two rows per group
+    ([2, 7, 40009], [1, 3, 2], 8, 0.3),
+    ([3, 50021, 11], [1, 3, 2], 2, 0.1),     # 6-byte units
+    ([3, 11, 50021], [1, 3, 2], 2, 0.1),
+    ([9, 70001, 5], [1, 3, 2], 1, 0.2),      # 9-byte units (1-byte words)
+    ([12, 5, 70001], [1, 3, 2], 1, 0.2),
+    ([3, 30011, 300], [1, 3, 2], 4, 0.05),   # m 300, runs of 8 units
+]
+
+
+def test_staged_row_tiles(torch_cuda):
+    # at the tensor's own address and one element past it (rows and scratch
+    # tiles then start at other offsets mod 16)
+    import eithot_amd as E
+    from oracle import pyoracle as O
+    old = {k: E.set_tuning(k, v) for k, v in (("lower_mask", 2), ("regroup", 0))}
+    try:
+        for dims, perm, elem, alpha in STAGED_ROW_CASES:
+            ops, _ = E.plan_describe(dims, perm, elem, 1, alpha)
+            assert any(o["kind"] == 3 and o["staged"] and o["us"] == o["U"] for o in ops), (dims, ops)
+            got, want, peak = run_case(torch_cuda, dims, perm, elem, seed=31, alpha=alpha)
+            assert_bitexact(got, want, f"staged rows {dims} {perm} elem {elem}")
+            assert peak <= alpha * got.nbytes
+            vol = int(np.prod(dims))
+            host = O.fill(vol + 1, elem, 32, 1)
+            dev = torch_cuda.from_numpy(host.copy()).to("cuda:0")
+            print(f"case offset 1 dims={dims} perm={perm} elem={elem}", flush=True)
+            E.transpose(dev, 1, len(dims), dims, [p - 1 for p in perm], elem, 1, alpha)
+            torch_cuda.cuda.synchronize()
+            want = host.copy()
+            want[elem:] = O.permute(np.ascontiguousarray(host[elem:]), dims, [p - 1 for p in perm], elem)
+            assert_bitexact(dev.cpu().numpy(), want, f"staged rows offset {dims} {perm} elem {elem}")
+    finally:
+        for k, v in old.items():
+            E.set_tuning(k, v)
+
+
 def test_narrow_slice_row_passes(torch_cuda):
     # rows longer than LDS with a 5 % budget: each unit is moved in narrow
     # slices (1 or 2 bytes of an 8-byte unit) through LDS, or through scratch
