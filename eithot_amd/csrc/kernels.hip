@@ -1101,7 +1101,8 @@ inline int64_t stage_row_tile_lds(int64_t m, int64_t L, int64_t U, int64_t idx_b
 // Run length of k_stage_row_tile for a staged row pass: tiles of about
 // 16 KiB of LDS (units and their indices) and runs of at least 96 bytes,
 // within 48 KiB; 0 when no such tile fits (the pass then gathers unit by
-// unit).
+// unit).  Longer runs at the cost of larger tiles measured slower: 270-byte
+// runs in 43 KiB tiles took 4.49 ms where 120-byte runs in 19 KiB took 3.13.
 inline int64_t stage_row_tile_len(const Op& op, int64_t idx_bytes) {
   if (op.kind != OP_ROW || op.m < 1 || op.U < 1) return 0;
   int64_t L = std::max<int64_t>(16384 / (op.m * (op.U + idx_bytes)), (96 + op.U - 1) / op.U);
