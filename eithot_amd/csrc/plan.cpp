@@ -906,6 +906,7 @@ static int64_t* tuning_field(Tuning& t, const char* key) {
   if (!strcmp(key, "cycle_min_cell")) return &t.cycle_min_cell;
   if (!strcmp(key, "fuse_max_block")) return &t.fuse_max_block;
   if (!strcmp(key, "block_digits")) return &t.block_digits;
+  if (!strcmp(key, "block_pad")) return &t.block_pad;
   if (!strcmp(key, "cycle_word_cost")) return &t.cycle_word_cost;
   if (!strcmp(key, "threads")) return &t.threads;
   if (!strcmp(key, "threads_big")) return &t.threads_big;
@@ -940,7 +941,8 @@ int tuning_set(const char* key, int64_t value) {
   if (f == &t.lower_mask && (value == 0 || value > LOWER_ALL)) return EITHOT_ERR_INVALID;
   if (f == &t.cycle_u_depth && value != 4 && value != 6 && value != 8) return EITHOT_ERR_INVALID;
   const bool flag = f == &t.regroup || f == &t.lds_threads_auto || f == &t.cycle_cell_pref ||
-                    f == &t.cycle_word_cost || f == &t.block_digits || f == &t.xcd_remap;
+                    f == &t.cycle_word_cost || f == &t.block_digits || f == &t.block_pad ||
+                    f == &t.xcd_remap;
   if (flag && value > 1) return EITHOT_ERR_INVALID;
   *f = value;
   t.epoch = g_tune.epoch + 1;
@@ -965,6 +967,7 @@ void tuning_apply(const Tuning& t, DeviceProfile& prof) {
   prof.cycle_min_cell = t.cycle_min_cell;
   prof.fuse_max_block = std::min<int64_t>(t.fuse_max_block, prof.lds_max);
   prof.block_digits = t.block_digits;
+  prof.block_pad = t.block_pad;
   prof.cycle_word_cost = t.cycle_word_cost;
   prof.reorder = (int)t.reorder;
   prof.regroup = t.regroup;
@@ -999,7 +1002,7 @@ namespace {
 bool fusable_tile(const Op& o) { return o.kind == OP_TILE && !o.staged && o.us == o.U; }
 
 std::shared_ptr<BlockTable> compose_tiles(const std::vector<int64_t>& parts, int64_t F, int64_t g,
-                                          bool digits) {
+                                          bool digits, bool pad) {
   auto t = std::make_shared<BlockTable>();
   const int64_t n = F / g;
   t->cells = n;
@@ -1053,7 +1056,7 @@ std::shared_ptr<BlockTable> compose_tiles(const std::vector<int64_t>& parts, int
   // LDS padding: the gather reads V = 16/g cells per thread (one word
   // each) across 64 lanes; pick the pad interval with the fewest bank
   // conflicts (64 banks of 4 bytes, and 32-lane halves on 32 banks)
-  if (g <= 16 && 64 % g == 0) {
+  if (pad && g <= 16 && 64 % g == 0) {
     const int64_t V = g < 16 ? 16 / g : 1;
     auto cost = [&](int pl) {
       double tot = 0;
@@ -1714,7 +1717,7 @@ static void finalize_plan(Plan& plan, double alpha, int64_t reserve, const Devic
   plan.table_bytes = 0;
   for (Op& op : plan.ops) {
     if (op.kind == OP_BLOCK && !op.blk)
-      op.blk = compose_tiles(op.bparts, op.R * op.U, op.U, prof.block_digits != 0);
+      op.blk = compose_tiles(op.bparts, op.R * op.U, op.U, prof.block_digits != 0, prof.block_pad != 0);
     if (op.kind == OP_CYCLE && !op.cyc) {
       op.cyc = choose_cycle_table(op.cmaps, op.R * op.C, op.B, op.U, budget, prof.cycle_lmax);
     }

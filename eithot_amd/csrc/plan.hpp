@@ -189,6 +189,7 @@ struct DeviceProfile {
   int64_t cycle_min_cell = 128;   // smallest cell moved by cycle following (0: off)
   int64_t fuse_max_block = 64 * 1024;  // largest fused block permutation (0: no fusion)
   int64_t block_digits = 1;       // fused blocks index cells arithmetically when they can
+  int64_t block_pad = 1;          // fused blocks pad LDS against bank conflicts (0: never)
   int64_t cycle_word_cost = 1;    // cycle passes of cells in narrow words cost more
   int reorder = 0;                // 0 cheapest of pre/post (mix breaks ties), 1 pre, 2 post, 3 mix
   int64_t regroup = 1;            // with reorder 0: also plan cell pass / tile / cell pass
@@ -210,6 +211,7 @@ struct Tuning {
   int64_t cycle_min_cell = 128;
   int64_t fuse_max_block = 64 * 1024;
   int64_t block_digits = 1;
+  int64_t block_pad = 1;
   int64_t cycle_word_cost = 1;
   int64_t threads = 256;
   int64_t threads_big = 512;

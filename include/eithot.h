@@ -208,6 +208,8 @@ int eithot_plan_segments(const int64_t* dims, const int* perm, int order, size_t
  *   "fuse_max_block" largest block of a fused tile permutation (65536; 0 = off)
  *   "block_digits"   fused blocks compute source cells from digits (1) or read
  *                    a gather table (0)
+ *   "block_pad"      fused blocks pad their LDS copy where a bank-conflict
+ *                    model says it pays (1) or never (0)
  *   "cycle_variant"  cycle-following kernel shape (0..7; 1..3 only for cells of
  *                    at least 64 words).  0 (default) picks: shifted 16-byte
  *                    chunks for cells that are not whole 16-byte words,
