@@ -2447,12 +2447,14 @@ struct BlkArgs {
   uint32_t P;
 };
 
-// source cell of output cell o: digit form, or the gather table
+// source cell of output cell o: digit form (at most ND digits), or the
+// gather table
+template <int ND>
 __device__ __forceinline__ uint32_t blk_src(const BlkArgs& a, uint32_t o) {
   if (a.ndig) {
     uint32_t q = o, v = 0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
+    for (int k = 0; k < ND; ++k) {
       if (k >= (int)a.ndig) break;
       const uint32_t nq = a.frad[k].div(q);
       v += (q - nq * a.frad[k].d) * a.stride[k];
@@ -2485,7 +2487,7 @@ __device__ __forceinline__ uint32_t blk_lds(const BlkArgs& a, uint32_t k, uint32
 // time and stored with one 16-byte store (with pattern offsets when P
 // threads divide the workgroup, else through the digit odometer); the run's
 // two edge chunks are stored word by word.
-template <typename Word, int V>
+template <typename Word, int V, int ND>
 __device__ __forceinline__ void block_run(const BlkArgs& a, const uint64_t f0, const uint32_t nf,
                                           unsigned char* lds) {
   constexpr uint32_t w = sizeof(Word);
@@ -2529,7 +2531,7 @@ __device__ __forceinline__ void block_run(const BlkArgs& a, const uint64_t f0, c
     const uint32_t r = wi - k * per;
     const uint32_t o = a.wpc == 1 ? r : a.fwpc.div(r);
     const uint32_t x = r - o * a.wpc;
-    const uint32_t src = blk_src(a, o);
+    const uint32_t src = blk_src<ND>(a, o);
     return *reinterpret_cast<const Word*>(
         lds + lds_off(sh + k * (uint32_t)a.F + (src * a.wpc + x) * w, a.pad));
   };
@@ -2548,7 +2550,7 @@ __device__ __forceinline__ void block_run(const BlkArgs& a, const uint64_t f0, c
     for (int v = 0; v < V; ++v) {
       const uint32_t c = rel / w + (uint32_t)v;  // cell from the period's start
       const uint32_t kb = a.fn.div(c);
-      off[v] = lds_off(sh + kb * (uint32_t)a.F + blk_src(a, c - kb * a.n) * w, a.pad);
+      off[v] = lds_off(sh + kb * (uint32_t)a.F + blk_src<ND>(a, c - kb * a.n) * w, a.pad);
     }
     // period of chunk t (-1 for the run's first chunk when it is partial)
     int32_t pi = (int32_t)(tid / a.P) - ((pos == 0 && sh > 0) ? 1 : 0);
@@ -2581,9 +2583,9 @@ __device__ __forceinline__ void block_run(const BlkArgs& a, const uint64_t f0, c
     const uint32_t r = w0 - k * per;
     uint32_t o = a.wpc == 1 ? r : a.fwpc.div(r);
     uint32_t x = r - o * a.wpc;
-    DigitOdometer od;
+    DigitOdometerN<ND> od;
     if (a.ndig) od.init(a.frad, a.stride, a.ndig, o);
-    else od.src = blk_src(a, o);
+    else od.src = blk_src<ND>(a, o);
     Packed<Word, V> tmp;
     if (a.wpc == 1 && a.ndig && a.frad[0].d >= (uint32_t)V) {
       // one-word cells whose fastest output digit spans at least V cells:
@@ -2621,11 +2623,11 @@ __device__ __forceinline__ void block_run(const BlkArgs& a, const uint64_t f0, c
             o = 0;
             ++k;
             if (a.ndig) od.init(a.frad, a.stride, a.ndig, 0);
-            else od.src = blk_src(a, 0);
+            else od.src = blk_src<ND>(a, 0);
           } else if (a.ndig) {
             od.step(a.frad, a.stride, a.ndig);
           } else {
-            od.src = blk_src(a, o);
+            od.src = blk_src<ND>(a, o);
           }
         }
       }
@@ -2635,13 +2637,13 @@ __device__ __forceinline__ void block_run(const BlkArgs& a, const uint64_t f0, c
 }
 
 // f0: first block of the workgroup's K blocks
-template <typename Word, int V>
+template <typename Word, int V, int ND>
 __device__ __forceinline__ void block_body(const BlkArgs& a, const uint64_t f0, unsigned char* lds) {
   constexpr uint32_t w = sizeof(Word);
   const uint32_t nf = (uint32_t)((a.B - f0) < a.K ? (a.B - f0) : a.K);
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   if (a.dense) {
-    block_run<Word, V>(a, f0, nf, lds);
+    block_run<Word, V, ND>(a, f0, nf, lds);
     return;
   }
   // blocks of whole 16-byte chunks at an aligned address: the workgroup's nf
@@ -2688,7 +2690,7 @@ __device__ __forceinline__ void block_body(const BlkArgs& a, const uint64_t f0, 
     for (int v = 0; v < V; ++v) {
       const uint32_t c = pos * V + (uint32_t)v;  // cell of the period
       const uint32_t kb = a.fn.div(c);
-      const uint32_t src = blk_src(a, c - kb * a.n);
+      const uint32_t src = blk_src<ND>(a, c - kb * a.n);
       off[v] = kb * a.Fl + lds_off(src * w, a.pad);
     }
     uint32_t pi = tid / a.P;  // period of chunk t
@@ -2714,9 +2716,9 @@ __device__ __forceinline__ void block_body(const BlkArgs& a, const uint64_t f0, 
       const uint32_t r = w0 - k * per;
       uint32_t o = a.wpc == 1 ? r : a.fwpc.div(r);
       uint32_t x = r - o * a.wpc;
-      DigitOdometer od;
+      DigitOdometerN<ND> od;
       if (a.ndig) od.init(a.frad, a.stride, a.ndig, o);
-      else od.src = blk_src(a, o);
+      else od.src = blk_src<ND>(a, o);
       Packed<Word, V> tmp;
       if (a.wpc == 1 && a.ndig && a.frad[0].d >= (uint32_t)V) {
         // one-word cells whose fastest output digit spans at least V cells:
@@ -2755,11 +2757,11 @@ __device__ __forceinline__ void block_body(const BlkArgs& a, const uint64_t f0, 
             o = 0;
             ++k;
             if (a.ndig) od.init(a.frad, a.stride, a.ndig, 0);
-            else od.src = blk_src(a, 0);
+            else od.src = blk_src<ND>(a, 0);
           } else if (a.ndig) {
             od.step(a.frad, a.stride, a.ndig);
           } else {
-            od.src = blk_src(a, o);
+            od.src = blk_src<ND>(a, o);
           }
         }
       }
@@ -2773,17 +2775,17 @@ __device__ __forceinline__ void block_body(const BlkArgs& a, const uint64_t f0, 
       const uint32_t r = t - k * per;
       const uint32_t o = a.wpc == 1 ? r : a.fwpc.div(r);
       const uint32_t x = r - o * a.wpc;
-      const uint32_t src = blk_src(a, o);
+      const uint32_t src = blk_src<ND>(a, o);
       reinterpret_cast<Word*>(out)[(uint64_t)k * per + r] =
           *reinterpret_cast<const Word*>(lds + blk_lds(a, k, (src * a.wpc + x) * w));
     }
   }
 }
 
-template <typename Word, int V>
+template <typename Word, int V, int ND>
 __global__ __launch_bounds__(512) void k_block(BlkArgs a, int remap) {
   extern __shared__ __align__(16) unsigned char lds[];
-  block_body<Word, V>(a, (uint64_t)xcd_logical(blockIdx.x, gridDim.x, remap) * a.K, lds);
+  block_body<Word, V, ND>(a, (uint64_t)xcd_logical(blockIdx.x, gridDim.x, remap) * a.K, lds);
 }
 
 const void* block_table_device(const BlockTable& t, hipError_t* err) {
@@ -2818,11 +2820,19 @@ hipError_t launch_block_w(BlkArgs a, int vec, int64_t lds, int64_t groups, const
   }
   // pattern stores want a whole number of periods per workgroup's threads
   if (vec && V > 1 && a.P) threads = std::max<int>((int)a.P, threads / (int)a.P * (int)a.P);
-  if (vec && V > 1)
-    hipLaunchKernelGGL((k_block<Word, V>), dim3((uint32_t)groups), dim3(threads), (size_t)lds, stream,
+  // digit tables of at most 4 digits (nearly all) run the 4-digit kernels
+  const bool d4 = a.ndig <= 4;
+  if (vec && V > 1 && d4)
+    hipLaunchKernelGGL((k_block<Word, V, 4>), dim3((uint32_t)groups), dim3(threads), (size_t)lds, stream,
+                       a, cfg.xcd_remap);
+  else if (vec && V > 1)
+    hipLaunchKernelGGL((k_block<Word, V, 8>), dim3((uint32_t)groups), dim3(threads), (size_t)lds, stream,
+                       a, cfg.xcd_remap);
+  else if (d4)
+    hipLaunchKernelGGL((k_block<Word, 1, 4>), dim3((uint32_t)groups), dim3(threads), (size_t)lds, stream,
                        a, cfg.xcd_remap);
   else
-    hipLaunchKernelGGL((k_block<Word, 1>), dim3((uint32_t)groups), dim3(threads), (size_t)lds, stream,
+    hipLaunchKernelGGL((k_block<Word, 1, 8>), dim3((uint32_t)groups), dim3(threads), (size_t)lds, stream,
                        a, cfg.xcd_remap);
   return hipGetLastError();
 }
@@ -3296,7 +3306,7 @@ __device__ __forceinline__ void batch_body(const BatchOpDev& op, uint32_t f0, un
   }
 #define EH_BLK(WL, VF)                                                                      \
   case 80 + WL * 2 + VF:                                                                    \
-    block_body<typename WordOf<WL>::T, VF ? (16 >> WL) : 1>(                                \
+    block_body<typename WordOf<WL>::T, VF ? (16 >> WL) : 1, 8>(                                \
         *reinterpret_cast<const BlkArgs*>(op.raw), item.f0, lds);                           \
     break;
 
@@ -3729,8 +3739,10 @@ void cfg_word(int64_t bytes, hipError_t& e) {
   };
   set((const void*)k_tile_oop<Word, uint32_t, 1>);
   set((const void*)k_tile_oop<Word, uint64_t, 1>);
-  set((const void*)k_block<Word, 1>);
-  if (V > 1) set((const void*)k_block<Word, V>);
+  set((const void*)k_block<Word, 1, 4>);
+  set((const void*)k_block<Word, 1, 8>);
+  if (V > 1) set((const void*)k_block<Word, V, 4>);
+  if (V > 1) set((const void*)k_block<Word, V, 8>);
   constexpr int w = sizeof(Word);
   set((const void*)k_segtile<Word, 1>);
   if (w >= 2) set((const void*)k_segtile<Word, (w >= 2 ? 2 : 1)>);

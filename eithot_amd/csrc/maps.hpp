@@ -98,14 +98,17 @@ struct FD32 {
 // Source cell of a digit permutation, stepped cell by cell.  A cell index
 // o has mixed-radix digits d_k (radix frad[k].d, fastest first) and its
 // source is sum d_k * stride[k]; step() moves to o + 1 with one add per
-// digit that changes (an odometer), instead of ndig divisions.
-struct DigitOdometer {
-  uint32_t dg[8];
+// digit that changes (an odometer), instead of ndig divisions.  ND bounds
+// ndig at compile time: kernels instantiated for at most 4 digits keep the
+// radices of digits 4..7 out of their scalar registers.
+template <int ND>
+struct DigitOdometerN {
+  uint32_t dg[ND];
   uint32_t src;
   EH_HD void init(const FD32* frad, const uint32_t* stride, uint32_t ndig, uint32_t o) {
     src = 0;
 #pragma unroll
-    for (int d = 0; d < 8; ++d) {
+    for (int d = 0; d < ND; ++d) {
       dg[d] = 0;
       if (d >= (int)ndig) continue;
       const uint32_t nq = frad[d].div(o);
@@ -116,7 +119,7 @@ struct DigitOdometer {
   }
   EH_HD void step(const FD32* frad, const uint32_t* stride, uint32_t ndig) {
 #pragma unroll
-    for (int d = 0; d < 8; ++d) {
+    for (int d = 0; d < ND; ++d) {
       if (d >= (int)ndig) break;
       src += stride[d];
       if (++dg[d] < frad[d].d) break;
@@ -125,6 +128,7 @@ struct DigitOdometer {
     }
   }
 };
+using DigitOdometer = DigitOdometerN<8>;
 
 template <typename Idx>
 struct FDsel;
