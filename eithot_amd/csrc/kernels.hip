@@ -1733,10 +1733,13 @@ struct CycArgs {
   FD64 fchunks, fnseg;
 };
 
-// source position of cell position p: the pass's maps composed in gather order
+// source position of cell position p: the pass's maps composed in gather
+// order (NM bounds nmaps at compile time: kernels instantiated for fewer maps
+// keep the others' constants out of their scalar registers)
+template <int NM = kMaxCycleMaps>
 __device__ __forceinline__ uint32_t cyc_src(const CycArgs& a, uint32_t p) {
 #pragma unroll
-  for (int k = kMaxCycleMaps - 1; k >= 0; --k) {
+  for (int k = NM - 1; k >= 0; --k) {
     if (k >= (int)a.nmaps) continue;
     const CycMapDev& M = a.maps[k];
     const uint32_t b = M.fbc.div(p), q = p - b * M.bc;
@@ -2231,7 +2234,7 @@ __device__ __forceinline__ void cycu_store(char* data, Off X, Off x0, Off x1, co
 // boundary area) stays below 2^31, else int64_t.
 // gtid: the thread's index among all threads of the pass (the lanes of an
 // item are consecutive threads of one wave)
-template <typename Word, typename Off, int D>
+template <typename Word, typename Off, int D, int NM = kMaxCycleMaps>
 __device__ __forceinline__ void cycle_u_body(const CycArgs& a, const CycUArgs& u, const uint64_t gtid) {
   const int L = 1 << u.lanes_log2;
   const uint32_t lane = (uint32_t)(gtid & (uint64_t)(L - 1));
@@ -2296,7 +2299,7 @@ __device__ __forceinline__ void cycle_u_body(const CycArgs& a, const CycUArgs& u
     uint32_t q[D];
     uint32_t pp = p;
 #pragma unroll
-    for (int k = 0; k < D; ++k) q[k] = pp = cyc_src(a, pp);
+    for (int k = 0; k < D; ++k) q[k] = pp = cyc_src<NM>(a, pp);
     W16 v[D];
     Off s0[D];
     uint32_t sig[D];
@@ -2342,9 +2345,9 @@ __device__ __forceinline__ void cycle_u_body(const CycArgs& a, const CycUArgs& u
   }
 }
 
-template <typename Word, typename Off, int D>
+template <typename Word, typename Off, int D, int NM = kMaxCycleMaps>
 __global__ __launch_bounds__(256) void k_cycle_u(CycArgs a, CycUArgs u) {
-  cycle_u_body<Word, Off, D>(a, u, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+  cycle_u_body<Word, Off, D, NM>(a, u, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Item geometry of the shifted cycle kernel for cells of w-byte words: fills
@@ -2380,7 +2383,11 @@ hipError_t launch_cycle_u(CycArgs a, int depth, hipStream_t stream) {
   if (blocks >= (1ull << 31)) return hipErrorInvalidValue;
   const dim3 grid((uint32_t)blocks), block(256);
   if (off32) {
-    if (depth == 4) hipLaunchKernelGGL((k_cycle_u<Word, int32_t, 4>), grid, block, 0, stream, a, u);
+    // passes of one or two composed maps (most) at the default depth: the
+    // kernel instantiated for two maps
+    if (depth == 4 && a.nmaps <= 2)
+      hipLaunchKernelGGL((k_cycle_u<Word, int32_t, 4, 2>), grid, block, 0, stream, a, u);
+    else if (depth == 4) hipLaunchKernelGGL((k_cycle_u<Word, int32_t, 4>), grid, block, 0, stream, a, u);
     else if (depth == 6) hipLaunchKernelGGL((k_cycle_u<Word, int32_t, 6>), grid, block, 0, stream, a, u);
     else hipLaunchKernelGGL((k_cycle_u<Word, int32_t, 8>), grid, block, 0, stream, a, u);
   } else {
