@@ -2088,12 +2088,17 @@ hipError_t prepare_cycle_args(const Op& op, char* data, char* scratch, int64_t s
 // area) are gathered byte by byte.  The step order and hazards are those
 // of k_cycle: within a wave, every load of a batch completes before any of
 // its stores.
+// Items are lanes consecutive lanes of one wave, per_wave of them to a wave
+// (any lane count: an item's last lane only hands its chunk down, so the
+// chunk it receives from the next item's first lane is never used).
 // ---------------------------------------------------------------------------
 struct CycUArgs {
   uint64_t G;         // cell bytes per item
   uint64_t groups;    // items per (matrix, segment)
-  uint32_t lanes_log2;
+  uint32_t lanes;     // lanes per item, ceil(G / 16) + 2 (at most 64)
+  uint32_t per_wave;  // items per wave, 64 / lanes
   FD64 fgroups;
+  FD32 flanes;
 };
 
 // bytes sig .. sig+15 of the 32 bytes a:b: dwords q .. q+4 (q = sig / 4) of
@@ -2236,12 +2241,14 @@ __device__ __forceinline__ void cycu_store(char* data, Off X, Off x0, Off x1, co
 // item are consecutive threads of one wave)
 template <typename Word, typename Off, int D, int NM = kMaxCycleMaps>
 __device__ __forceinline__ void cycle_u_body(const CycArgs& a, const CycUArgs& u, const uint64_t gtid) {
-  const int L = 1 << u.lanes_log2;
-  const uint32_t lane = (uint32_t)(gtid & (uint64_t)(L - 1));
-  const uint64_t item = gtid >> u.lanes_log2;
-  // every lane of a live item takes part in the shuffles: an item is whole
-  // lanes of one wave (L divides 64), and items past the end exit together
-  if (item >= a.items) return;
+  const uint32_t lw = (uint32_t)(gtid & 63);
+  const uint32_t sub = u.flanes.div(lw);
+  const uint32_t lane = lw - sub * u.lanes;
+  const uint64_t item = (gtid >> 6) * u.per_wave + sub;
+  // every lane of a live item takes part in the shuffles: an item's lanes
+  // are consecutive lanes of one wave, and they exit together (lanes past
+  // the wave's last item and items past the end feed no live lane's store)
+  if (sub >= u.per_wave || item >= a.items) return;
   const uint64_t rest = u.fgroups.div(item);
   const uint64_t g = item - rest * u.groups;
   const uint64_t b = a.fnseg.div(rest);
@@ -2277,7 +2284,7 @@ __device__ __forceinline__ void cycle_u_body(const CycArgs& a, const CycUArgs& u
   // nothing
   auto assemble = [&](const W16& c, Off dr, Off s0, uint32_t sig, const char* sb, Off slo,
                       Off shi) -> W16 {
-    const W16 nx = shfl_next(c, L);
+    const W16 nx = shfl_next(c, 64);
     if (s0 >= slo && s0 + 32 <= shi) return sig ? funnel16(c, nx, sig) : c;
     if (chunk(dr) >= dr + n) return c;
     return gather16(sb, (int64_t)(s0 + (Off)sig), (int64_t)slo, (int64_t)shi);
@@ -2358,16 +2365,21 @@ __global__ __launch_bounds__(256) void k_cycle_u(CycArgs a, CycUArgs u) {
 // Returns whether every offset the kernel forms (within [-16, the end of the
 // pass or of the boundary area + 64 lanes' chunks + 32)) fits 32 bits.
 bool cycle_u_geometry(CycArgs& a, CycUArgs& u, uint64_t w) {
-  uint32_t ll = 2;
-  while (ll < 6 && ((uint64_t)((1u << ll) - 2) * 16) < a.U) ++ll;
-  u.lanes_log2 = ll;
-  const uint64_t gmax = (uint64_t)((1u << ll) - 2) * 16;
+  const uint64_t gmax = 62 * 16;
   u.groups = (a.U + gmax - 1) / gmax;
   u.G = ((a.U + u.groups - 1) / u.groups + w - 1) / w * w;
+  u.lanes = (uint32_t)((u.G + 15) / 16 + 2);
+  u.per_wave = 64 / u.lanes;
+  u.flanes.init(u.lanes);
   u.fgroups.init(u.groups);
   a.items = a.B * a.nseg * u.groups;
   const uint64_t span = std::max<uint64_t>(a.base + a.B * a.RCU, a.B * a.nslots * a.U) + 2048;
   return span < (1ull << 31);
+}
+
+// threads of a shifted cycle pass: whole waves of per_wave items each
+inline uint64_t cycu_threads(const CycArgs& a, const CycUArgs& u) {
+  return (a.items + u.per_wave - 1) / u.per_wave * 64;
 }
 
 template <typename Word>
@@ -2379,7 +2391,7 @@ hipError_t launch_cycle_u(CycArgs a, int depth, hipStream_t stream) {
   }
   CycUArgs u;
   const bool off32 = cycle_u_geometry(a, u, sizeof(Word));
-  const uint64_t blocks = ((a.items << u.lanes_log2) + 255) / 256;
+  const uint64_t blocks = (cycu_threads(a, u) + 255) / 256;
   if (blocks >= (1ull << 31)) return hipErrorInvalidValue;
   const dim3 grid((uint32_t)blocks), block(256);
   if (off32) {
@@ -3521,7 +3533,7 @@ int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, cons
       if (shifted) {
         std::memcpy(d.raw, &cu, sizeof(cu));
         d.variant = 120 + log2i(wu);
-        groups = ((cu.a.items << cu.u.lanes_log2) + threads - 1) / threads;
+        groups = (cycu_threads(cu.a, cu.u) + threads - 1) / threads;
       } else {
         std::memcpy(d.raw, &a, sizeof(a));
         d.variant = 70 + log2i(w);

@@ -167,6 +167,47 @@ def test_cycle_following_passes(torch_cuda):
         E.set_tuning("cycle_word_cost", 1)
 
 
+# cells that are not whole 16-byte words, sized so the shifted cycle kernel
+# packs items of 3 to 64 lanes (ceil(cell / 16) + 2), 1 to 21 to a wave
+LANE_GROUP_CASES = [([5, 33, 29], [1, 3, 2], 2),      # 10 B: 3 lanes, 21 items per wave
+                       ([10, 33, 29], [1, 3, 2], 2),     # 20 B: 4 lanes
+                       ([9, 41, 23], [1, 3, 2], 4),      # 36 B: 5 lanes
+                       ([13, 47, 53], [1, 3, 2], 4),     # 52 B: 6 lanes
+                       ([45, 29, 31], [1, 3, 2], 2),     # 90 B: 8 lanes
+                       ([65, 40, 33], [1, 3, 2], 2),     # 130 B: 11 lanes, 5 items per wave
+                       ([35, 41, 29], [1, 3, 2], 4),     # 140 B: 11 lanes
+                       ([138, 23, 31], [1, 3, 2], 2),    # 276 B: 20 lanes
+                       ([125, 37, 19], [1, 3, 2], 4),    # 500 B: 34 lanes, one item per wave
+                       ([1100, 13, 11], [1, 3, 2], 1)]   # 1100 B: two groups of 64 lanes
+
+
+def test_shifted_cycle_lane_groups(torch_cuda):
+    # only cycle following allowed, down to 8-byte cells; at the tensor's
+    # address and one element past it
+    import eithot_amd as E
+    from oracle import pyoracle as O
+    knobs = (("lower_mask", 16), ("regroup", 0), ("cycle_min_cell", 8), ("cycle_word_cost", 0))
+    old = {k: E.set_tuning(k, v) for k, v in knobs}
+    try:
+        for dims, perm, elem in LANE_GROUP_CASES:
+            ops, _ = E.plan_describe(dims, perm, elem, 1, 0.05)
+            assert [o["kind"] for o in ops] == [4] and ops[0]["U"] % 16, (dims, ops)
+            got, want, _ = run_case(torch_cuda, dims, perm, elem, seed=33, alpha=0.05)
+            assert_bitexact(got, want, f"shifted cycle {dims} {perm} elem {elem}")
+            vol = int(np.prod(dims))
+            host = O.fill(vol + 1, elem, 34, 1)
+            dev = torch_cuda.from_numpy(host.copy()).to("cuda:0")
+            print(f"case offset 1 dims={dims} perm={perm} elem={elem}", flush=True)
+            E.transpose(dev, 1, len(dims), dims, [p - 1 for p in perm], elem, 1, 0.05)
+            torch_cuda.cuda.synchronize()
+            want = host.copy()
+            want[elem:] = O.permute(np.ascontiguousarray(host[elem:]), dims, [p - 1 for p in perm], elem)
+            assert_bitexact(dev.cpu().numpy(), want, f"shifted cycle offset {dims} {perm} elem {elem}")
+    finally:
+        for k, v in old.items():
+            E.set_tuning(k, v)
+
+
 def test_composite_cycle_passes(torch_cuda):
     # consecutive whole-cell passes composed into one cycle-following pass
     from test_planner import COMPOSITE_CASES
