@@ -2432,8 +2432,12 @@ hipError_t launch_cycle(const Op& op, char* data, char* scratch, int64_t scratch
   // cells of at least 512 bytes of 16-byte words: non-temporal loads and
   // stores (variant 5), measured 1.7 % faster on 512-byte cells and 2-7 %
   // on larger ones; smaller cells lose up to 70 % with them (neighbouring
-  // cells share cache lines that the hints evict)
+  // cells share cache lines that the hints evict).  Smaller cells in passes
+  // of at least 512 MiB: non-temporal stores only (variant 4), 2-6 % faster
+  // on 128-464-byte cells of 0.8-4 GB passes; a 231 MB pass, which the
+  // 256 MB last-level cache could hold, was 30 % slower with them
   if (variant == 0 && w == 16 && a.U >= 512) variant = 5;
+  else if (variant == 0 && w == 16 && a.B * a.RCU >= (512ull << 20)) variant = 4;
   switch (w) {
     case 16: return launch_cycle_w<W16>(a, variant, stream);
     case 8: return launch_cycle_w<uint64_t>(a, variant, stream);

@@ -214,7 +214,9 @@ int eithot_plan_segments(const int64_t* dims, const int* perm, int order, size_t
  *                    at least 64 words).  0 (default) picks: shifted 16-byte
  *                    chunks for cells that are not whole 16-byte words,
  *                    non-temporal loads and stores (as 5) for cells of at
- *                    least 512 bytes, else plain 16-byte words
+ *                    least 512 bytes, non-temporal stores (as 4) for
+ *                    smaller 16-byte-word cells in passes of at least
+ *                    512 MiB, else plain 16-byte words
  *   "cycle_lmax"     > 0: cycle segments of at most this many positions
  *                    (0: chosen per pass)
  *   "cycle_u_depth"  positions in flight per item when cycle following moves
