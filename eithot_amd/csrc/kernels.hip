@@ -1255,12 +1255,13 @@ int64_t word_bytes(const Op& op, uintptr_t data) {
   return w;
 }
 
-// workgroups want at least this many footprints in flight across the GPU
-constexpr int64_t kMinGroups = 1024;
-
-int64_t pick_K(int64_t nfp, int64_t fpb, int64_t lds_target) {
-  int64_t K = std::max<int64_t>(1, lds_target / std::max<int64_t>(fpb, 1));
-  if (nfp / K < kMinGroups) K = std::max<int64_t>(1, nfp / kMinGroups);
+// Footprints per workgroup: as many as fit the LDS target, but no fewer
+// than min_groups workgroups per pass (a pass launched alone wants at least
+// this many footprints in flight across the GPU)
+int64_t pick_K(int64_t nfp, int64_t fpb, const LaunchCfg& cfg) {
+  int64_t K = std::max<int64_t>(1, cfg.lds_target / std::max<int64_t>(fpb, 1));
+  const int64_t mg = std::max<int64_t>(1, cfg.min_groups);
+  if (nfp / K < mg) K = std::max<int64_t>(1, nfp / mg);
   return K;
 }
 
@@ -1424,7 +1425,7 @@ void prepare_lds(const Op& op, char* data, const LaunchCfg& cfg, Args<Idx>& a, P
   a.fvpr.init(a.vpr);
   // wide tiles: copies of the run of K matrices (two, or one for wide 3)
   const int copies = (a.wide && op.kind == OP_TILE) ? (a.wide == 3 ? 1 : 2) : 0;
-  p.K = pick_K((int64_t)a.nfp, (copies ? copies : 1) * (int64_t)a.fpl, cfg.lds_target);
+  p.K = pick_K((int64_t)a.nfp, (copies ? copies : 1) * (int64_t)a.fpl, cfg);
   if (a.wide && op.kind == OP_ROW)  // the run of K rows and its two partial chunks
     while (p.K > 1 && p.K * (int64_t)a.fpl + 32 > kWideLdsMax) --p.K;
   if (copies)
@@ -2912,7 +2913,7 @@ hipError_t prepare_block_args(const Op& op, char* data, const LaunchCfg& cfg, Bl
               ? (uint32_t)Pc
               : 0;
   }
-  int64_t K = pick_K(op.B, a.dense ? Fl : (int64_t)a.Fl, cfg.lds_target);
+  int64_t K = pick_K(op.B, a.dense ? Fl : (int64_t)a.Fl, cfg);
   while (K > 1 && lds_of(K) > 160 * 1024) --K;
   // a multiple of the blocks that make 16-byte runs and whole 16-byte stores
   // (past the LDS target when that many blocks fit the workgroup's LDS)
@@ -3170,7 +3171,7 @@ hipError_t prepare_seg_args(const Op& op, char* data, const LaunchCfg& cfg, SegA
   const int64_t fpl = (op.R * rs + 15) / 16 * 16;
   const int64_t nseg = bytes / op.segb;
   if (fpl + nseg * 8 > 160 * 1024) return hipErrorInvalidValue;
-  int64_t K = pick_K(op.B, fpl + nseg * 8, cfg.lds_target);
+  int64_t K = pick_K(op.B, fpl + nseg * 8, cfg);
   while (K > 1 && K * (fpl + nseg * 8) > 160 * 1024) --K;
   std::memset(&a, 0, sizeof(a));
   a.data = data;
@@ -3470,10 +3471,14 @@ int64_t batch_boundary_bytes(const Op& op) {
   return ((int64_t)op.B * (int64_t)op.cyc->slots.size() * op.U + 255) / 256 * 256;
 }
 
-int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, const LaunchCfg& cfg,
+int batch_launch_build(const std::vector<std::pair<const Op*, char*>>& ops, const LaunchCfg& cfg_in,
                        int grid_per_cu, int cus, char* scratch, int64_t scratch_bytes,
                        BatchLaunch** out, std::vector<size_t>* rejected) {
   *out = nullptr;
+  // the batch's passes share each launch: footprints pack to the LDS target
+  // instead of spreading every pass over the whole GPU
+  LaunchCfg cfg = cfg_in;
+  cfg.min_groups = 1;
   // workgroup size: the larger one when any LDS footprint needs it
   bool big = false;
   for (const auto& pr : ops) {

@@ -23,6 +23,9 @@ struct LaunchCfg {
   int lds_threads_auto = 1;        // 1: tile/column/row workgroups sized for one load round
   int segtile_nt = 3;              // non-temporal segmented tile loads (bit 0) and stores (bit 1)
   int cycle_u_depth = 4;           // positions in flight per item of the shifted cycle kernel
+  int64_t min_groups = 1024;       // workgroups a pass's footprints spread over at least (fewer
+                                   // footprints per workgroup below that); batched launches,
+                                   // whose passes share the GPU, use 1
 };
 
 // One pass of `op` over the tensor at `data`, on `stream`.  Staged ops use
