@@ -924,6 +924,7 @@ static int64_t* tuning_field(Tuning& t, const char* key) {
   if (!strcmp(key, "lower_mask")) return &t.lower_mask;
   if (!strcmp(key, "hold_floor")) return &t.hold_floor;
   if (!strcmp(key, "cycle_u_depth")) return &t.cycle_u_depth;
+  if (!strcmp(key, "min_groups")) return &t.min_groups;
   return nullptr;
 }
 
@@ -942,6 +943,7 @@ int tuning_set(const char* key, int64_t value) {
   if (f == &t.segtile_nt && value != 0 && value != 2 && value != 3) return EITHOT_ERR_INVALID;
   if (f == &t.lower_mask && (value == 0 || value > LOWER_ALL)) return EITHOT_ERR_INVALID;
   if (f == &t.cycle_u_depth && value != 4 && value != 6 && value != 8) return EITHOT_ERR_INVALID;
+  if (f == &t.min_groups && (value < 1 || value > (1 << 20))) return EITHOT_ERR_INVALID;
   const bool flag = f == &t.regroup || f == &t.lds_threads_auto || f == &t.cycle_cell_pref ||
                     f == &t.cycle_word_cost || f == &t.block_digits || f == &t.block_pad ||
                     f == &t.xcd_remap;

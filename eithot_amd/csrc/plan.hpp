@@ -227,6 +227,7 @@ struct Tuning {
   int64_t lower_mask = LOWER_ALL;
   int64_t hold_floor = 64ll << 20;  // device bytes the library may keep whatever alpha allows
   int64_t cycle_u_depth = 4;        // positions in flight per item of the shifted cycle kernel (4, 6, 8)
+  int64_t min_groups = 1024;        // workgroups a single-launch pass spreads its footprints over
   uint64_t epoch = 0;  // bumped on every change (plan caches key on it)
 };
 Tuning tuning_get();

@@ -96,6 +96,7 @@ LaunchCfg Device::launch_cfg() const {
   c.lds_threads_auto = (int)t.lds_threads_auto;
   c.segtile_nt = (int)t.segtile_nt;
   c.cycle_u_depth = (int)t.cycle_u_depth;
+  c.min_groups = t.min_groups;
   return c;
 }
 

@@ -221,6 +221,9 @@ int eithot_plan_segments(const int64_t* dims, const int* perm, int order, size_t
  *                    (0: chosen per pass)
  *   "cycle_u_depth"  positions in flight per item when cycle following moves
  *                    cells that are not whole 16-byte words (4 default, 6, 8)
+ *   "min_groups"     workgroups a pass launched on its own spreads its LDS
+ *                    footprints over, at least (1024; 1..2^20); batched
+ *                    launches pack footprints to the LDS target instead
  *   "cycle_cell_pref" cycle passes of cells of 256 B and more cost less in
  *                    the planner (0 default: not faster on MI355X so far)
  *   "cycle_word_cost" cycle passes over cells that are not whole 16-byte

@@ -51,11 +51,12 @@ def test_tuning_rejects_out_of_range_values():
     import eithot_amd as E
     for key, bad in (("segtile_nt", 1), ("regroup", 2), ("lds_threads_auto", 5),
                      ("cycle_variant", 8), ("threads", 96), ("cycle_lmax", 1),
-                     ("cycle_u_depth", 5), ("cycle_u_depth", 16), ("block_pad", 2)):
+                     ("cycle_u_depth", 5), ("cycle_u_depth", 16), ("block_pad", 2),
+                     ("min_groups", 0)):
         with pytest.raises(E.EithotError):
             E.set_tuning(key, bad)
     for key, good in (("segtile_nt", 2), ("regroup", 0), ("cycle_lmax", 64), ("cycle_u_depth", 8),
-                      ("block_pad", 0)):
+                      ("block_pad", 0), ("min_groups", 256)):
         old = E.set_tuning(key, good)
         try:
             assert E.get_tuning(key) == good
