@@ -221,7 +221,7 @@ std::vector<hipEvent_t>& timing_events(size_t n) {
 }
 
 int run_plan(Device* d, const Plan& plan, char* data, char* scratch, int64_t scratch_bytes,
-             hipStream_t stream, std::vector<float>* times) {
+             hipStream_t stream, std::vector<float>* times, int64_t min_groups_cap) {
   std::vector<hipEvent_t> ev;
   if (times) {
     const std::vector<hipEvent_t>& pool = timing_events(plan.ops.size() + 1);
@@ -233,7 +233,8 @@ int run_plan(Device* d, const Plan& plan, char* data, char* scratch, int64_t scr
     }
   }
   int rc = EITHOT_OK;
-  const LaunchCfg cfg = d->launch_cfg();
+  LaunchCfg cfg = d->launch_cfg();
+  if (min_groups_cap > 0) cfg.min_groups = std::min(cfg.min_groups, min_groups_cap);
   for (size_t i = 0; i < plan.ops.size(); ++i) {
     hipError_t e = launch_op(plan.ops[i], data, scratch, scratch_bytes, cfg, stream);
     if (e != hipSuccess) {

@@ -70,7 +70,10 @@ Device* device_ctx(int* err);
 int fail(int code, const char* fmt, ...);
 void set_peak(size_t bytes);
 int validate(const void* dev, const int64_t* dims, const int* perm, int order, size_t elem);
+// Enqueues the plan's passes on `stream`.  min_groups_cap > 0 lowers the
+// launch configuration's workgroup floor per pass to at most that (callers
+// whose passes share the GPU with other work).
 int run_plan(Device* d, const Plan& plan, char* data, char* scratch, int64_t scratch_bytes,
-             hipStream_t stream, std::vector<float>* times);
+             hipStream_t stream, std::vector<float>* times, int64_t min_groups_cap = 0);
 
 }  // namespace eithot
