@@ -250,6 +250,10 @@ struct Plan {
   int64_t device_table_bytes(int dev) const;
   // Frees this plan's pass tables on device `dev` (uploaded again on next use).
   void drop_device_tables(int dev) const;
+  // The runtime's launch choices learned on this plan's calls (the
+  // segmented tile's block order), created on its first run; the runtime
+  // serialises access to it.
+  mutable std::shared_ptr<void> launch_state;
 };
 
 // Full plan.  perm1 is 1-based.  alpha bounds the scratch (scratch <= alpha *
