@@ -223,13 +223,15 @@ std::vector<hipEvent_t>& timing_events(size_t n) {
 // The segmented tile's block order (xcd_remap 1: each XCD takes a
 // contiguous range of matrices; 0: the hardware's round-robin) wins by 3-7 %
 // one way or the other depending on the shape, with no static rule found.
-// A plan's first call runs each segmented-tile pass with order 1 and its
-// second with order 0, both between timing events; a later call whose
-// events have completed keeps the faster order for good.  The calls do
-// their real work, so the trial costs no extra transposes.
+// A plan's first call runs each segmented-tile pass with order 1 untimed
+// (it may be cold); its second runs order 1 and its third order 0, both
+// between timing events; a later call whose events have completed keeps the
+// faster order for good.  The calls do their real work, so the trial costs
+// no extra transposes.
 namespace {
 struct SegOrderTrial {
-  int stage = 0;   // 0: time order 1 next, 1: time order 0 next, 2: both timed, 3: decided
+  int stage = -1;  // -1: untimed first call, 0: time order 1 next, 1: time order 0 next,
+                   // 2: both timed, 3: decided
   int choice = 1;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   ~SegOrderTrial() {
@@ -274,7 +276,10 @@ void seg_orders(const Plan& plan, int base_remap, hipStream_t stream, std::vecto
   for (size_t i = 0; i < plan.ops.size(); ++i) {
     if (plan.ops[i].kind != OP_SEGTILE) continue;
     SegOrderTrial& t = st->seg[i];
-    if (t.stage == 0) {
+    if (t.stage == -1) {
+      remap[i] = 1;
+      t.stage = 0;
+    } else if (t.stage == 0) {
       bool ok = true;
       for (hipEvent_t& e : t.ev)
         if (!e && hipEventCreate(&e) != hipSuccess) ok = false;

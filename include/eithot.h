@@ -236,10 +236,11 @@ int eithot_plan_segments(const int64_t* dims, const int* perm, int order, size_t
  *   "lds_threads_auto" tile, column, row and block workgroups sized for one round of
  *                    8 loads per thread (1) or from "threads"/"threads_big" (0)
  *   "xcd_remap"      keep consecutive footprints on one XCD (1); with 1, a
- *                    plan's first two calls run its segmented-tile passes in
- *                    each block order (XCD-contiguous, then round-robin)
- *                    between timing events and later calls keep the faster
- *                    (not under graph capture); 0 forces round-robin
+ *                    plan's second and third calls run its segmented-tile
+ *                    passes in each block order (XCD-contiguous, then
+ *                    round-robin) between timing events and later calls
+ *                    keep the faster (not under graph capture); 0 forces
+ *                    round-robin
  *   "hold_floor"     device bytes (arena and pass tables) the library may keep
  *                    after a call even when alpha * tensor bytes is smaller
  *                    (64 MiB; 0: alpha alone bounds them)
